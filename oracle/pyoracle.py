@@ -1,0 +1,260 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/libgpe_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product package
+(gpesolver_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgpe_oracle.so")
+
+POT = {
+    "HARMONIC": 0,
+    "MEXICAN": 1,
+    "MEXICAN_FREE": 2,
+    "CYLINDRICAL": 3,
+    "FREE": 4,
+    "MEXICAN_ASYMETRIC": 5,
+    "MEXICAN_ASYMETRIC_FREE": 6,
+}
+INIT = {"COS": 0, "GAUSS": 1, "MULTIPLE_GAUSS": 2, "SETUP_GAUSS": 3}
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("m", C.c_double), ("n_atoms", C.c_double),
+        ("omega_x", C.c_double), ("omega_y", C.c_double), ("omega_z", C.c_double),
+        ("dd", C.c_double),
+        ("edd", C.c_double), ("edd_start", C.c_double), ("edd_stop", C.c_double),
+        ("const_edd", C.c_int),
+        ("iter_imag", C.c_long), ("iter_real", C.c_long),
+        ("imag_time_dt", C.c_double), ("real_time_dt", C.c_double),
+        ("aa", C.c_double), ("b", C.c_double), ("dxdydz", C.c_double), ("add", C.c_double),
+        ("cdd", C.c_double), ("ggp11", C.c_double), ("gamma", C.c_double), ("w_15", C.c_double),
+    ]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        vp = C.c_void_p
+        L.orc_solver_create.restype = vp
+        L.orc_solver_create.argtypes = [C.POINTER(OrcParams)]
+        L.orc_solver_destroy.argtypes = [vp]
+        L.orc_solver_params.restype = C.POINTER(OrcParams)
+        L.orc_solver_params.argtypes = [vp]
+        for name in ("orc_psi", "orc_psii", "orc_fi3d", "orc_pote", "orc_vdip_table", "orc_kinetic_table"):
+            getattr(L, name).restype = dp
+            getattr(L, name).argtypes = [vp]
+        L.orc_xnorma.restype = C.c_double
+        L.orc_xnorma.argtypes = [vp]
+        for name in ("orc_calc_fi3d", "orc_calc_norm", "orc_normalize", "orc_imag_iter_linear_step",
+                     "orc_imag_iter_nonlinear_step", "orc_real_fft_potential_half_step",
+                     "orc_real_fft_kinetic_step", "orc_imag_time_iter", "orc_real_time_iter",
+                     "orc_refresh_derived"):
+            getattr(L, name).argtypes = [vp]
+        L.orc_load_buffer.argtypes = [vp, dp]
+        L.orc_load_pote.argtypes = [vp, dp]
+        L.orc_calc_energy.argtypes = [vp, dp]
+        L.orc_it_steps.argtypes = [vp, C.c_long]
+        L.orc_rt_steps.argtypes = [vp, C.c_long, C.c_long]
+        L.orc_init_parameters.argtypes = [C.POINTER(OrcParams)]
+        L.orc_update_edd.argtypes = [C.POINTER(OrcParams), C.c_long]
+        L.orc_fill_potential.argtypes = [C.c_int, C.POINTER(OrcParams), dp]
+        L.orc_init_wavefunction.restype = C.c_int
+        L.orc_init_wavefunction.argtypes = [C.c_int, C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int,
+                                            C.c_int, C.c_double, dp]
+        L.orc_fft_1d.argtypes = [dp, C.c_int, C.c_int]
+        L.orc_fft_3d.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
+        for name in ("orc_len_nm_to_au", "orc_len_au_to_nm", "orc_mass_Da_to_au", "orc_freq_Hz_to_au",
+                     "orc_ene_au_to_meV"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_params(nx, ny, nz, dx_nm, dy_nm, dz_nm, m_da=163.929, n_atoms=2e4, edd=1.5,
+                omega_hz=(60.0, 60.0, 120.0), dd_nm=1500.0, edd_start=None, edd_stop=None,
+                iter_imag=0, iter_real=0, imag_dt=1.25e11, real_dt=1.0e10) -> OrcParams:
+    """Parameters as FileManager::load_all_v1 produces them (file_manager.cpp:436-548)."""
+    L = lib()
+    p = OrcParams()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.dx, p.dy, p.dz = (L.orc_len_nm_to_au(v) for v in (dx_nm, dy_nm, dz_nm))
+    p.m = L.orc_mass_Da_to_au(m_da)
+    p.n_atoms = n_atoms
+    p.omega_x, p.omega_y, p.omega_z = (L.orc_freq_Hz_to_au(v) for v in omega_hz)
+    p.dd = L.orc_len_nm_to_au(dd_nm)
+    if edd_start is not None:
+        p.const_edd = 0
+        p.edd_start, p.edd_stop = edd_start, edd_stop
+        p.edd = edd if edd is not None else edd_start
+    else:
+        p.const_edd = 1
+        p.edd = edd
+    p.iter_imag, p.iter_real = iter_imag, iter_real
+    p.imag_time_dt, p.real_time_dt = imag_dt, real_dt
+    L.orc_init_parameters(C.byref(p))
+    return p
+
+
+def fft3(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    nx, ny, nz = a.shape
+    lib().orc_fft_3d(_dp(a.view(np.float64)), nx, ny, nz, sign)
+    return a
+
+
+def fft1(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().orc_fft_1d(_dp(a.view(np.float64)), a.shape[0], sign)
+    return a
+
+
+def potential(key: str, p: OrcParams) -> np.ndarray:
+    V = np.zeros((p.nx, p.ny, p.nz), dtype=np.float64)
+    lib().orc_fill_potential(POT[key], C.byref(p), _dp(V))
+    return V
+
+
+def wavefunction(strategy: str, p: OrcParams, n_gauss_max=2, drops=(3, 3, 2), sigma_cells=0.0) -> np.ndarray:
+    psi = np.zeros((p.nx, p.ny, p.nz), dtype=np.complex128)
+    rc = lib().orc_init_wavefunction(INIT[strategy], C.byref(p), n_gauss_max, drops[0], drops[1], drops[2],
+                                     sigma_cells, _dp(psi.view(np.float64)))
+    if rc != 0:
+        raise ValueError("bad init strategy arguments")
+    return psi
+
+
+class Oracle:
+    """The oracle solver: one CpuGrossPitaevskiSolver state (cpu_solver.cpp)."""
+
+    def __init__(self, p: OrcParams):
+        self.L = lib()
+        self.h = self.L.orc_solver_create(C.byref(p))
+        self.shape = (p.nx, p.ny, p.nz)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_solver_destroy(self.h)
+            self.h = None
+
+    @property
+    def params(self) -> OrcParams:
+        return self.L.orc_solver_params(self.h).contents
+
+    def _view(self, ptr, dtype, shape):
+        n = int(np.prod(shape)) * (2 if dtype == np.complex128 else 1)
+        arr = np.ctypeslib.as_array(ptr, shape=(n,))
+        return arr.view(dtype).reshape(shape)
+
+    @property
+    def psi(self) -> np.ndarray:
+        return self._view(self.L.orc_psi(self.h), np.complex128, self.shape)
+
+    @property
+    def psii(self) -> np.ndarray:
+        return self._view(self.L.orc_psii(self.h), np.complex128, self.shape)
+
+    @property
+    def fi3d(self) -> np.ndarray:
+        return self._view(self.L.orc_fi3d(self.h), np.float64, self.shape)
+
+    @property
+    def pote(self) -> np.ndarray:
+        return self._view(self.L.orc_pote(self.h), np.float64, self.shape)
+
+    @property
+    def xnorma(self) -> float:
+        return self.L.orc_xnorma(self.h)
+
+    def vdip(self) -> np.ndarray:
+        nx, ny, nz = self.shape
+        return self._view(self.L.orc_vdip_table(self.h), np.float64, (2 * nx, 2 * ny, nz + 1)).copy()
+
+    def kinetic(self) -> np.ndarray:
+        return self._view(self.L.orc_kinetic_table(self.h), np.float64, self.shape).copy()
+
+    def refresh(self):
+        self.L.orc_refresh_derived(self.h)
+
+    def load_buffer(self, psi: np.ndarray):
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        self.L.orc_load_buffer(self.h, _dp(a.view(np.float64)))
+
+    def load_pote(self, V: np.ndarray):
+        a = np.ascontiguousarray(V, dtype=np.float64)
+        self.L.orc_load_pote(self.h, _dp(a))
+
+    def calc_fi3d(self):
+        self.L.orc_calc_fi3d(self.h)
+
+    def calc_norm(self):
+        self.L.orc_calc_norm(self.h)
+
+    def normalize(self):
+        self.L.orc_normalize(self.h)
+
+    def imag_linear(self):
+        self.L.orc_imag_iter_linear_step(self.h)
+
+    def imag_nonlinear(self):
+        self.L.orc_imag_iter_nonlinear_step(self.h)
+
+    def potential_half_step(self):
+        self.L.orc_real_fft_potential_half_step(self.h)
+
+    def kinetic_step(self):
+        self.L.orc_real_fft_kinetic_step(self.h)
+
+    def it_step(self):
+        self.L.orc_imag_time_iter(self.h)
+
+    def rt_step(self):
+        self.L.orc_real_time_iter(self.h)
+
+    def it_steps(self, n: int):
+        self.L.orc_it_steps(self.h, n)
+
+    def rt_steps(self, first_iter: int, n: int):
+        self.L.orc_rt_steps(self.h, first_iter, n)
+
+    def energy(self) -> dict:
+        out = np.zeros(8)
+        self.L.orc_calc_energy(self.h, _dp(out))
+        keys = ("e_kin", "e_pot", "e_int", "e_ext", "e_bmf", "e_total", "mu", "residual")
+        return dict(zip(keys, out.tolist()))
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(n)
