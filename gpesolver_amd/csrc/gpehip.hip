@@ -1,0 +1,1233 @@
+// gpehip.hip -- MI355X (gfx950) Gross-Pitaevskii solver backend: kernels + C ABI.
+//
+// Data layout in HBM (one solver instance, grid nx x ny x nz, N = nx*ny*nz):
+//   store[2]  complex T [nx][ny][nz]   phi = sqrt(dV) * psi (ping-pong for IT)
+//   psii      complex T [nx][ny][nz]   the reference's cpsii (its boundary is
+//                                      what every IT step writes there)
+//   V, fi3d   T [nx][ny][nz]           potential and dipolar potential (a.u.)
+//   S1        complex T [2ny][nx][nz+1]  Poisson spectrum after the z (R2C)
+//                                      and y passes, x still in real space
+//   Tk        complex T [ny][nx][nz]   kinetic split-step work array
+// The field is stored scaled, phi = sqrt(dV) psi, so that sum |phi|^2 = the
+// reference's xnorma and no FP32 value becomes subnormal (|psi|^5 ~ 1e-42 in
+// a.u. would); the nonlinear couplings absorb the powers of dV.
+//
+// Kernels (T = float or double):
+//   k_plane  one block per x-plane i: finishes the dipolar Poisson solve of the
+//            plane (inverse y FFT, z C2R), applies the IT stencil or the RT
+//            potential phase, starts the next Poisson solve (z R2C, y FFT of
+//            N|psi|^2) and runs the kinetic propagation along z and y.
+//   k_xpass  one block per tile of x-lines: Poisson x FFT, x Vdd(k), inverse
+//            x FFT on S1, and the kinetic propagation along x on Tk.
+// One IT or RT step is one k_xpass plus one k_plane launch.  The kinetic
+// propagator exp(-i dt k^2/2m) = prod_ax exp(-i dt k_ax^2/2m) is separable, so
+// the 3D split step is three 1D (FFT, phase, inverse FFT) passes; the Poisson
+// multiplier kz^2/k^2 is not separable and keeps the 3-stage pipeline.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpehip.h"
+#include "gpe_fft.hpp"
+
+using gpe::cplx;
+using gpe::LinePlan;
+
+namespace {
+
+constexpr int kPlaneThreads = 512;
+constexpr int kLineThreads = 256;
+constexpr int kMaxPlanePts = 8192;  // ny*nz bound of the single-block plane path
+constexpr int kLineMaxE = 4096;     // complex elements of one x-line tile
+
+template <typename T>
+struct Lim;
+template <>
+struct Lim<float> {
+    static constexpr int maxe = 20224;  // 158 KiB of complex<float> (static LDS uses the rest)
+};
+template <>
+struct Lim<double> {
+    static constexpr int maxe = 10112;  // 158 KiB of complex<double>
+};
+
+// Mode bits of k_plane.
+enum : int {
+    M_PINV = 1,        // finish Poisson from S1 -> Phi (registers)
+    M_PHI_LOAD = 2,    // Phi from the fi3d array
+    M_PHI_STORE = 4,   // write Phi to the fi3d array
+    M_IN_TK = 8,       // psi plane comes from Tk (else from `in`, canonical layout)
+    M_PHASE_HALF = 16, // exp(-i dt/2 Veff) on the interior
+    M_PHASE_FULL = 32, // two merged half steps (end of one step + start of next)
+    M_RHO_FWD = 64,    // start Poisson of |psi|^2 -> S1
+    M_KIN = 128,       // z,y kinetic propagation, result to Tk
+    M_OUT = 256,       // write psi plane to `out` (canonical)
+    M_IT = 512,        // imaginary-time stencil from `in` to `out`
+    M_NORM_OUT = 1024, // per-plane sum |psi_out|^2 -> partials_out[i]
+    M_NORM_IN = 2048,  // scale input by 1/sqrt(sum partials_in)
+};
+
+template <typename T>
+struct PlaneArgs {
+    int nx, ny, nz, mode;
+    const cplx<T>* in;
+    cplx<T>* out;
+    const cplx<T>* psii;
+    cplx<T>* tk;
+    cplx<T>* s1;
+    const T* V;
+    T* phi;
+    const double* part_in;
+    double* part_out;
+    const cplx<T>* tw;
+    const cplx<T>* ez;  // exp(i fz(k)) / nz
+    const cplx<T>* ey;  // exp(i fy(j)) / ny
+    LinePlan pz2, py2, pz, py;
+    double pinv_scale;  // (N_atoms / dV) / (8 N)
+    double cdd;
+    // IT stencil (phi units)
+    double dtau, cx, cy, cz, gq, gmq;
+    // RT phase: theta = -hdt * (cv*(V + cdd*Phi) + pg*rho + pgm*rho^1.5)
+    double hdt, cv, pg, pgm;
+};
+
+template <typename T>
+struct LineArgs {
+    int nx, ny, nz;
+    int ntile_p;  // Poisson tiles (first blocks)
+    int cp;       // kz per Poisson tile
+    int ck;       // k per kinetic tile
+    int tiles_kz;  // ceil((nz+1)/cp)
+    int tiles_k;   // ceil(nz/ck)
+    cplx<T>* s1;
+    cplx<T>* tk;
+    const cplx<T>* tw;
+    const cplx<T>* ex;  // exp(i fx(i)) / nx
+    const double* kx2;  // padded lattice kx^2, [2nx]
+    const double* ky2;  // padded lattice ky^2, [2ny]
+    const double* kz2;  // padded lattice kz^2, [nz+1]
+    LinePlan px2, px;
+};
+
+template <typename T>
+__device__ __forceinline__ void sincos_t(T a, T* s, T* c);
+template <>
+__device__ __forceinline__ void sincos_t<float>(float a, float* s, float* c) {
+    sincosf(a, s, c);
+}
+template <>
+__device__ __forceinline__ void sincos_t<double>(double a, double* s, double* c) {
+    sincos(a, s, c);
+}
+
+// Deterministic block sum of one double per thread (fixed tree).
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < NT / 64; ++w) r += red[w];
+        red[NT / 64] = r;
+    }
+    __syncthreads();
+    r = red[NT / 64];
+    __syncthreads();
+    return r;
+}
+
+// Sum of the per-plane partials, same order in every block.
+__device__ double sum_partials(const double* p, int n, double* red) {
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += p[i];
+        red[0] = s;
+    }
+    __syncthreads();
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Plane kernel.  Dynamic LDS: max(2ny*(nz+1), ny*nz) complex T.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
+    constexpr int MAXE = Lim<T>::maxe;
+    constexpr int PPT = kMaxPlanePts / NT;  // points per thread
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* P = reinterpret_cast<cplx<T>*>(lds_raw);
+    __shared__ double red[NT / 64 + 1];
+
+    const int i = blockIdx.x;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int nzh = nz + 1, nz2 = 2 * nz, ny2 = 2 * ny;
+    const int npl = ny * nz;
+    const int mode = a.mode;
+    const size_t plane_off = (size_t)i * npl;
+
+    T phi_r[PPT];
+    T rho_r[PPT];
+
+    double s_in = 1.0;
+    if (mode & M_NORM_IN) s_in = 1.0 / sqrt(sum_partials(a.part_in, nx, red));
+
+    // ---- 1. Phi for this plane -------------------------------------------
+    if (mode & M_PINV) {
+        // load S1[ky][i][kz] -> P[ky][kz]
+        for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
+            const int ky = e / nzh, kz = e - ky * nzh;
+            P[e] = a.s1[((size_t)ky * nx + i) * nzh + kz];
+        }
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(P, a.py2, nzh, nzh, 1, +1, a.tw);
+        // rows j < ny -> Z[j][0..2nz) with Hermitian extension, via registers
+        constexpr int RQ = (MAXE / 2 + NT - 1) / NT;
+        cplx<T> tmp[RQ];
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int e = threadIdx.x + q * NT;
+            if (e < ny * nzh) tmp[q] = P[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int e = threadIdx.x + q * NT;
+            if (e < ny * nzh) {
+                const int j = e / nzh, kz = e - j * nzh;
+                P[j * nz2 + kz] = tmp[q];
+                if (kz >= 1 && kz < nz) P[j * nz2 + (nz2 - kz)] = gpe::mk(tmp[q].x, -tmp[q].y);
+            }
+        }
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(P, a.pz2, ny, 1, nz2, +1, a.tw);
+        const T sc = (T)(a.pinv_scale * s_in * s_in);
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < npl) {
+                const int j = p / nz, k = p - j * nz;
+                phi_r[q] = P[j * nz2 + k].x * sc;
+                if (mode & M_PHI_STORE) a.phi[plane_off + p] = phi_r[q];
+            }
+        }
+        __syncthreads();
+    } else if (mode & M_PHI_LOAD) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < npl) phi_r[q] = a.phi[plane_off + p];
+        }
+    }
+
+    // ---- 2. psi update: IT stencil or RT phase ------------------------------
+    double nacc = 0.0;
+    const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
+    cplx<T>* Q = P;  // aliases P: P is free here
+    if (mode & M_IT) {
+        const cplx<T>* in = a.in;
+        for (int p = threadIdx.x; p < npl; p += NT) {
+            cplx<T> v = in[plane_off + p];
+            Q[p] = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+        }
+        __syncthreads();
+        const T sv = (T)s_in;
+        const T dtau = (T)a.dtau, cx = (T)a.cx, cy = (T)a.cy, cz = (T)a.cz, gq = (T)a.gq, gmq = (T)a.gmq,
+                cdd = (T)a.cdd;
+        const bool iint = (i >= 1 && i <= nx - 2);
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < npl) {
+                const int j = p / nz, k = p - j * nz;
+                cplx<T> r;
+                if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
+                    const cplx<T> c = Q[p];
+                    const cplx<T> xm = in[plane_off - npl + p], xp = in[plane_off + npl + p];
+                    const cplx<T> ym = Q[p - nz], yp = Q[p + nz], zm = Q[p - 1], zp = Q[p + 1];
+                    const T v = a.V[plane_off + p] + cdd * phi_r[q];
+                    cplx<T> c1;
+                    c1.x = cx * (xm.x * sv + xp.x * sv - T(2) * c.x) + cy * (ym.x + yp.x - T(2) * c.x) +
+                           cz * (zm.x + zp.x - T(2) * c.x) + c.x * v;
+                    c1.y = cx * (xm.y * sv + xp.y * sv - T(2) * c.y) + cy * (ym.y + yp.y - T(2) * c.y) +
+                           cz * (zm.y + zp.y - T(2) * c.y) + c.y * v;
+                    const T nrm = c.x * c.x + c.y * c.y;
+                    const T ab = sqrt(nrm);
+                    const T f = gq * nrm + gmq * (ab * ab * ab);
+                    r.x = c.x - dtau * c1.x;
+                    r.y = c.y - dtau * c1.y;
+                    r.x = r.x - dtau * (f * c.x);
+                    r.y = r.y - dtau * (f * c.y);
+                } else {
+                    r = a.psii[plane_off + p];
+                }
+                a.out[plane_off + p] = r;
+                const T n2 = r.x * r.x + r.y * r.y;
+                rho_r[q] = n2;
+                nacc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+            }
+        }
+        __syncthreads();
+    } else if (have_psi) {
+        // load psi plane into Q
+        if (mode & M_IN_TK) {
+            for (int p = threadIdx.x; p < npl; p += NT) {
+                const int j = p / nz, k = p - j * nz;
+                Q[p] = a.tk[((size_t)j * nx + i) * nz + k];
+            }
+        } else {
+            for (int p = threadIdx.x; p < npl; p += NT) {
+                cplx<T> v = a.in[plane_off + p];
+                if (mode & M_NORM_IN) v = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+                Q[p] = v;
+            }
+        }
+        __syncthreads();
+        const bool iint = (i >= 1 && i <= nx - 2);
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < npl) {
+                cplx<T> c = Q[p];
+                T nrm = c.x * c.x + c.y * c.y;
+                if (mode & (M_PHASE_HALF | M_PHASE_FULL)) {
+                    const int j = p / nz, k = p - j * nz;
+                    if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
+                        const T ab = sqrt(nrm);
+                        const T veff = (T)a.cv * (a.V[plane_off + p] + (T)a.cdd * phi_r[q]) + (T)a.pg * nrm +
+                                       (T)a.pgm * (ab * ab * ab);
+                        T sn, cs;
+                        sincos_t<T>(-(T)a.hdt * veff, &sn, &cs);
+                        c = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
+                        Q[p] = c;
+                        nrm = c.x * c.x + c.y * c.y;
+                    }
+                }
+                rho_r[q] = nrm;
+                nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
+            }
+        }
+        __syncthreads();
+        if (mode & M_OUT) {
+            for (int p = threadIdx.x; p < npl; p += NT) a.out[plane_off + p] = Q[p];
+        }
+        if (mode & M_KIN) {
+            // z propagation: rows j, length nz
+            gpe::lds_fft<T, NT, MAXE>(Q, a.pz, ny, 1, nz, -1, a.tw);
+            for (int p = threadIdx.x; p < npl; p += NT) Q[p] = gpe::cmul(Q[p], a.ez[p % nz]);
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(Q, a.pz, ny, 1, nz, +1, a.tw);
+            // y propagation: columns k, length ny
+            gpe::lds_fft<T, NT, MAXE>(Q, a.py, nz, nz, 1, -1, a.tw);
+            for (int p = threadIdx.x; p < npl; p += NT) Q[p] = gpe::cmul(Q[p], a.ey[p / nz]);
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(Q, a.py, nz, nz, 1, +1, a.tw);
+            for (int p = threadIdx.x; p < npl; p += NT) {
+                const int j = p / nz, k = p - j * nz;
+                a.tk[((size_t)j * nx + i) * nz + k] = Q[p];
+            }
+        }
+        __syncthreads();
+    }
+
+    if (mode & M_NORM_OUT) {
+        const double s = block_sum<NT>(nacc, red);
+        if (threadIdx.x == 0) a.part_out[i] = s;
+    }
+
+    // ---- 3. start the Poisson solve of rho = |psi|^2 ----------------------
+    if (mode & M_RHO_FWD) {
+        // Z[j][k] = rho for k < nz, 0 for k in [nz, 2nz)
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < npl) {
+                const int j = p / nz, k = p - j * nz;
+                P[j * nz2 + k] = gpe::mk(rho_r[q], T(0));
+                P[j * nz2 + nz + k] = gpe::mk(T(0), T(0));
+            }
+        }
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(P, a.pz2, ny, 1, nz2, -1, a.tw);
+        constexpr int RQ = (MAXE / 2 + NT - 1) / NT;
+        cplx<T> tmp[RQ];
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int e = threadIdx.x + q * NT;
+            if (e < ny * nzh) {
+                const int j = e / nzh, kz = e - j * nzh;
+                tmp[q] = P[j * nz2 + kz];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int e = threadIdx.x + q * NT;
+            if (e < ny * nzh) P[e] = tmp[q];
+        }
+        for (int e = ny * nzh + threadIdx.x; e < ny2 * nzh; e += NT) P[e] = gpe::mk(T(0), T(0));
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(P, a.py2, nzh, nzh, 1, -1, a.tw);
+        for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
+            const int ky = e / nzh, kz = e - ky * nzh;
+            a.s1[((size_t)ky * nx + i) * nzh + kz] = P[e];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Line kernel: x-direction passes.  Dynamic LDS: max(2nx*cp, nx*ck) complex T.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
+    constexpr int MAXE = kLineMaxE;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
+    const int nx = a.nx, nzh = a.nz + 1;
+    int b = blockIdx.x;
+    if (b < a.ntile_p) {
+        const int ky = b / a.tiles_kz;
+        const int kz0 = (b - ky * a.tiles_kz) * a.cp;
+        const int C = min(a.cp, nzh - kz0);
+        const int nx2 = 2 * nx;
+        cplx<T>* base = a.s1 + (size_t)ky * nx * nzh + kz0;
+        for (int e = threadIdx.x; e < nx2 * C; e += NT) {
+            const int ii = e / C, c = e - ii * C;
+            X[e] = (ii < nx) ? base[(size_t)ii * nzh + c] : gpe::mk(T(0), T(0));
+        }
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(X, a.px2, C, C, 1, -1, a.tw);
+        const double ky2 = a.ky2[ky];
+        for (int e = threadIdx.x; e < nx2 * C; e += NT) {
+            const int ii = e / C, c = e - ii * C;
+            const double kz2 = a.kz2[kz0 + c];
+            const double k2 = a.kx2[ii] + ky2 + kz2;
+            const T v = (k2 > 1e-30) ? (T)(kz2 / k2) : T(0);
+            X[e] = gpe::cscale(X[e], v);
+        }
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(X, a.px2, C, C, 1, +1, a.tw);
+        for (int e = threadIdx.x; e < nx * C; e += NT) {
+            const int ii = e / C, c = e - ii * C;
+            base[(size_t)ii * nzh + c] = X[e];
+        }
+    } else {
+        b -= a.ntile_p;
+        const int nz = a.nz;
+        const int j = b / a.tiles_k;
+        const int k0 = (b - j * a.tiles_k) * a.ck;
+        const int C = min(a.ck, nz - k0);
+        cplx<T>* base = a.tk + (size_t)j * nx * nz + k0;
+        for (int e = threadIdx.x; e < nx * C; e += NT) {
+            const int ii = e / C, c = e - ii * C;
+            X[e] = base[(size_t)ii * nz + c];
+        }
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(X, a.px, C, C, 1, -1, a.tw);
+        for (int e = threadIdx.x; e < nx * C; e += NT) X[e] = gpe::cmul(X[e], a.ex[e / C]);
+        __syncthreads();
+        gpe::lds_fft<T, NT, MAXE>(X, a.px, C, C, 1, +1, a.tw);
+        for (int e = threadIdx.x; e < nx * C; e += NT) {
+            const int ii = e / C, c = e - ii * C;
+            base[(size_t)ii * nz + c] = X[e];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise / reduction kernels.
+
+// psi *= s over every point, s = 1/sqrt(sum partials) (normalize).
+template <typename T>
+__global__ void k_scale(cplx<T>* psi, size_t n, const double* part, int np) {
+    __shared__ double sred[2];
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int t = 0; t < np; ++t) s += part[t];
+        sred[0] = 1.0 / sqrt(s);
+    }
+    __syncthreads();
+    const T sc = (T)sred[0];
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        cplx<T> v = psi[e];
+        psi[e] = gpe::mk(v.x * sc, v.y * sc);
+    }
+}
+
+// Per-plane sum |psi|^2 (calc_norm), one block per plane.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_norm_planes(const cplx<T>* psi, int npl, double* part) {
+    __shared__ double red[NT / 64 + 1];
+    const cplx<T>* p = psi + (size_t)blockIdx.x * npl;
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < npl; e += NT) {
+        cplx<T> v = p[e];
+        acc += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    const double s = block_sum<NT>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// RT potential half step, interior only (cpu_solver.cpp:99-131).
+template <typename T>
+__global__ void k_potential_half(cplx<T>* psi, const T* V, const T* phi, int nx, int ny, int nz, double hdt,
+                                 double cdd, double pg, double pgm) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int k = e % nz;
+        const int j = (e / nz) % ny;
+        const int i = e / ((size_t)ny * nz);
+        if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > nz - 2) continue;
+        cplx<T> c = psi[e];
+        const T nrm = c.x * c.x + c.y * c.y;
+        const T ab = sqrt(nrm);
+        const T veff = V[e] + (T)cdd * phi[e] + (T)pg * nrm + (T)pgm * (ab * ab * ab);
+        T sn, cs;
+        sincos_t<T>(-(T)hdt * veff, &sn, &cs);
+        psi[e] = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
+    }
+}
+
+// Energies (cpu_solver.cpp:137-194) and <psi|H|psi>, per plane, FP64 sums.
+// pass 0: out[i*8 + {0..5}] = {sum Re(-Lap psi conj psi) (no 1/2m), sum V|phi|^2,
+//          sum |phi|^4, sum Phi|phi|^2, sum |phi|^5, sum Re(conj phi H phi)}
+// pass 1: out[i*8 + 6] = sum |H phi - mu phi|^2
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_energy(const cplx<T>* psi, const T* V, const T* phi, int nx, int ny, int nz,
+                                               double ix2, double iy2, double iz2, double m, double cdd, double gq,
+                                               double gmq, double mu, int pass, double* out) {
+    __shared__ double red[NT / 64 + 1];
+    const int i = blockIdx.x;
+    const int npl = ny * nz;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (i >= 1 && i <= nx - 2) {
+        const size_t off = (size_t)i * npl;
+        for (int p = threadIdx.x; p < npl; p += NT) {
+            const int j = p / nz, k = p - j * nz;
+            if (j < 1 || j > ny - 2 || k < 1 || k > nz - 2) continue;
+            const size_t e = off + p;
+            const cplx<T> cT = psi[e];
+            const double cr = cT.x, ci = cT.y;
+            auto ld = [&](size_t t, double& r, double& im) {
+                cplx<T> v = psi[t];
+                r = v.x;
+                im = v.y;
+            };
+            double xmr, xmi, xpr, xpi, ymr, ymi, ypr, ypi, zmr, zmi, zpr, zpi;
+            ld(e - npl, xmr, xmi);
+            ld(e + npl, xpr, xpi);
+            ld(e - nz, ymr, ymi);
+            ld(e + nz, ypr, ypi);
+            ld(e - 1, zmr, zmi);
+            ld(e + 1, zpr, zpi);
+            // -Lap psi
+            const double lr = -(xmr + xpr - 2 * cr) * ix2 - (ymr + ypr - 2 * cr) * iy2 - (zmr + zpr - 2 * cr) * iz2;
+            const double li = -(xmi + xpi - 2 * ci) * ix2 - (ymi + ypi - 2 * ci) * iy2 - (zmi + zpi - 2 * ci) * iz2;
+            const double nrm = cr * cr + ci * ci;
+            const double ab = sqrt(nrm);
+            const double v = (double)V[e];
+            const double ph = (double)phi[e];
+            const double f = v + cdd * ph + gq * nrm + gmq * nrm * ab;
+            const double hr = lr / (2 * m) + f * cr;
+            const double hi = li / (2 * m) + f * ci;
+            if (pass == 0) {
+                acc[0] += lr * cr + li * ci;
+                acc[1] += v * nrm;
+                acc[2] += nrm * nrm;
+                acc[3] += ph * nrm;
+                acc[4] += nrm * nrm * ab;
+                acc[5] += cr * hr + ci * hi;
+            } else {
+                const double rr = hr - mu * cr, ri = hi - mu * ci;
+                acc[0] += rr * rr + ri * ri;
+            }
+        }
+    }
+    const int nv = pass == 0 ? 6 : 1;
+    for (int t = 0; t < nv; ++t) {
+        const double s = block_sum<NT>(acc[t], red);
+        if (threadIdx.x == 0) out[(size_t)i * 8 + (pass == 0 ? t : 6)] = s;
+    }
+}
+
+// Unfused IT step (the two hooks imag_iter_linear_step + imag_iter_nonlinear_step):
+// out interior from in (already normalised), out boundary from psii.
+template <typename T>
+__global__ void k_it_stencil(const cplx<T>* in, cplx<T>* out, const cplx<T>* psii, const T* V, const T* phi, int nx,
+                             int ny, int nz, double dtau_, double cx_, double cy_, double cz_, double cdd_, double gq_,
+                             double gmq_) {
+    const size_t n = (size_t)nx * ny * nz;
+    const int npl = ny * nz;
+    const T dtau = (T)dtau_, cx = (T)cx_, cy = (T)cy_, cz = (T)cz_, cdd = (T)cdd_, gq = (T)gq_, gmq = (T)gmq_;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int k = e % nz;
+        const int j = (e / nz) % ny;
+        const int i = e / ((size_t)npl);
+        if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > nz - 2) {
+            out[e] = psii[e];
+            continue;
+        }
+        const cplx<T> c = in[e];
+        const cplx<T> xm = in[e - npl], xp = in[e + npl], ym = in[e - nz], yp = in[e + nz], zm = in[e - 1],
+                      zp = in[e + 1];
+        const T v = V[e] + cdd * phi[e];
+        cplx<T> c1;
+        c1.x = cx * (xm.x + xp.x - T(2) * c.x) + cy * (ym.x + yp.x - T(2) * c.x) + cz * (zm.x + zp.x - T(2) * c.x) +
+               c.x * v;
+        c1.y = cx * (xm.y + xp.y - T(2) * c.y) + cy * (ym.y + yp.y - T(2) * c.y) + cz * (zm.y + zp.y - T(2) * c.y) +
+               c.y * v;
+        const T nrm = c.x * c.x + c.y * c.y;
+        const T ab = sqrt(nrm);
+        const T f = gq * nrm + gmq * (ab * ab * ab);
+        cplx<T> r;
+        r.x = c.x - dtau * c1.x;
+        r.y = c.y - dtau * c1.y;
+        r.x = r.x - dtau * (f * c.x);
+        r.y = r.y - dtau * (f * c.y);
+        out[e] = r;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+#define GPE_CHECK(call)                                                                             \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            return fail(h, std::string(#call) + ": " + hipGetErrorString(e_));                      \
+        }                                                                                           \
+    } while (0)
+
+struct gpe_handle {
+    gpe_params p;
+    int prec;
+    size_t N;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // device buffers (bytes allocated per precision)
+    void* store[2] = {nullptr, nullptr};
+    int cur = 0;
+    void* psii = nullptr;
+    void* V = nullptr;
+    void* phi = nullptr;
+    void* s1 = nullptr;
+    void* tk = nullptr;
+    void* tw = nullptr;
+    void* ex = nullptr;
+    void* ey = nullptr;
+    void* ez = nullptr;
+    double* kx2 = nullptr;
+    double* ky2 = nullptr;
+    double* kz2 = nullptr;
+    double* part[2] = {nullptr, nullptr};
+    double* epart = nullptr;
+    int pcur = 0;
+    bool norm_pending = false;  // true: normalised psi = store[cur]/sqrt(sum part[pcur])
+    LinePlan px, py, pz, px2, py2, pz2;
+    // launch shapes
+    int cp = 0, ck = 0;
+    size_t plane_lds = 0, line_lds = 0;
+    // scaled constants
+    double sqdv = 1.0;
+    double ggp11, gamma;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double t_plane_us = 0, t_line_us = 0, t_total_ms = 0;
+};
+
+static std::string g_create_err;
+
+static int fail(gpe_handle* h, const std::string& msg) {
+    if (h)
+        h->err = msg;
+    else
+        g_create_err = msg;
+    return 1;
+}
+
+static bool make_plan(int L, LinePlan& pl) {
+    pl.L = L;
+    pl.nst = 0;
+    int r = L;
+    while (r % 4 == 0 && pl.nst < gpe::kMaxStages) {
+        pl.radix[pl.nst++] = 4;
+        r /= 4;
+    }
+    while (r % 2 == 0 && pl.nst < gpe::kMaxStages) {
+        pl.radix[pl.nst++] = 2;
+        r /= 2;
+    }
+    while (r % 3 == 0 && pl.nst < gpe::kMaxStages) {
+        pl.radix[pl.nst++] = 3;
+        r /= 3;
+    }
+    return r == 1;
+}
+
+static size_t csize(const gpe_handle* h) { return h->prec == GPE_FP64 ? 16 : 8; }
+static size_t rsize(const gpe_handle* h) { return h->prec == GPE_FP64 ? 8 : 4; }
+
+// Host FP64 array -> device array of the handle's precision (optionally scaled).
+static int upload_real(gpe_handle* h, void* dst, const double* src, size_t n, double scale) {
+    if (h->prec == GPE_FP64) {
+        std::vector<double> b(n);
+        for (size_t t = 0; t < n; ++t) b[t] = src[t] * scale;
+        GPE_CHECK(hipMemcpyAsync(dst, b.data(), n * 8, hipMemcpyHostToDevice, h->stream));
+        GPE_CHECK(hipStreamSynchronize(h->stream));
+    } else {
+        std::vector<float> b(n);
+        for (size_t t = 0; t < n; ++t) b[t] = (float)(src[t] * scale);
+        GPE_CHECK(hipMemcpyAsync(dst, b.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        GPE_CHECK(hipStreamSynchronize(h->stream));
+    }
+    return 0;
+}
+
+static int download_real(gpe_handle* h, double* dst, const void* src, size_t n, double scale) {
+    if (h->prec == GPE_FP64) {
+        std::vector<double> b(n);
+        GPE_CHECK(hipMemcpyAsync(b.data(), src, n * 8, hipMemcpyDeviceToHost, h->stream));
+        GPE_CHECK(hipStreamSynchronize(h->stream));
+        for (size_t t = 0; t < n; ++t) dst[t] = b[t] * scale;
+    } else {
+        std::vector<float> b(n);
+        GPE_CHECK(hipMemcpyAsync(b.data(), src, n * 4, hipMemcpyDeviceToHost, h->stream));
+        GPE_CHECK(hipStreamSynchronize(h->stream));
+        for (size_t t = 0; t < n; ++t) dst[t] = (double)b[t] * scale;
+    }
+    return 0;
+}
+
+template <typename T>
+static void fill_phase(std::vector<cplx<T>>& out, int n, double d, double dt, double m) {
+    out.resize(n);
+    for (int t = 0; t < n; ++t) {
+        const double k = (t <= n / 2) ? 2.0 * M_PI * t / (n * d) : 2.0 * M_PI * (t - n) / (n * d);
+        const double f = -dt / (2.0 * m) * k * k;
+        out[t].x = (T)(cos(f) / n);
+        out[t].y = (T)(sin(f) / n);
+    }
+}
+
+template <typename T>
+static int setup_tables(gpe_handle* h) {
+    const gpe_params& p = h->p;
+    // twiddles for the six line lengths
+    const int lens[6] = {2 * p.nx, 2 * p.ny, 2 * p.nz, p.nx, p.ny, p.nz};
+    LinePlan* pls[6] = {&h->px2, &h->py2, &h->pz2, &h->px, &h->py, &h->pz};
+    std::vector<cplx<T>> tw;
+    for (int t = 0; t < 6; ++t) {
+        if (!make_plan(lens[t], *pls[t])) return fail(h, "grid length " + std::to_string(lens[t]) +
+                                                             " is not of the form 2^a 3^b");
+        pls[t]->tw = (int)tw.size();
+        for (int m = 0; m < lens[t]; ++m) {
+            const double ang = 2.0 * M_PI * m / lens[t];
+            cplx<T> c;
+            c.x = (T)cos(ang);
+            c.y = (T)(-sin(ang));
+            tw.push_back(c);
+        }
+    }
+    GPE_CHECK(hipMalloc(&h->tw, tw.size() * sizeof(cplx<T>)));
+    GPE_CHECK(hipMemcpy(h->tw, tw.data(), tw.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    // kinetic phase tables (fftw_split_solver.cpp:11-42, factorised per axis)
+    std::vector<cplx<T>> e;
+    fill_phase<T>(e, p.nx, p.dx, p.real_time_dt, p.m);
+    GPE_CHECK(hipMalloc(&h->ex, e.size() * sizeof(cplx<T>)));
+    GPE_CHECK(hipMemcpy(h->ex, e.data(), e.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    fill_phase<T>(e, p.ny, p.dy, p.real_time_dt, p.m);
+    GPE_CHECK(hipMalloc(&h->ey, e.size() * sizeof(cplx<T>)));
+    GPE_CHECK(hipMemcpy(h->ey, e.data(), e.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    fill_phase<T>(e, p.nz, p.dz, p.real_time_dt, p.m);
+    GPE_CHECK(hipMalloc(&h->ez, e.size() * sizeof(cplx<T>)));
+    GPE_CHECK(hipMemcpy(h->ez, e.data(), e.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    // padded-lattice k^2 per axis (fftw_poisson_solver.cpp:12-49), FP64
+    const int NX = 2 * p.nx, NY = 2 * p.ny, NZ = 2 * p.nz;
+    std::vector<double> kx(NX), ky(NY), kz(p.nz + 1);
+    const double dkx = 2. * M_PI / (NX * p.dx), dky = 2. * M_PI / (NY * p.dy), dkz = 2. * M_PI / (NZ * p.dz);
+    for (int t = 0; t < NX; ++t) {
+        const double k = (t <= NX / 2) ? t * dkx : (t - NX) * dkx;
+        kx[t] = k * k;
+    }
+    for (int t = 0; t < NY; ++t) {
+        const double k = (t <= NY / 2) ? t * dky : (t - NY) * dky;
+        ky[t] = k * k;
+    }
+    for (int t = 0; t <= p.nz; ++t) {
+        const double k = t * dkz;
+        kz[t] = k * k;
+    }
+    GPE_CHECK(hipMalloc(&h->kx2, NX * 8));
+    GPE_CHECK(hipMalloc(&h->ky2, NY * 8));
+    GPE_CHECK(hipMalloc(&h->kz2, (p.nz + 1) * 8));
+    GPE_CHECK(hipMemcpy(h->kx2, kx.data(), NX * 8, hipMemcpyHostToDevice));
+    GPE_CHECK(hipMemcpy(h->ky2, ky.data(), NY * 8, hipMemcpyHostToDevice));
+    GPE_CHECK(hipMemcpy(h->kz2, kz.data(), (p.nz + 1) * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
+    PlaneArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    const gpe_params& p = h->p;
+    const double dV = p.dx * p.dy * p.dz;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.mode = mode;
+    a.psii = (const cplx<T>*)h->psii;
+    a.tk = (cplx<T>*)h->tk;
+    a.s1 = (cplx<T>*)h->s1;
+    a.V = (const T*)h->V;
+    a.phi = (T*)h->phi;
+    a.tw = (const cplx<T>*)h->tw;
+    a.ez = (const cplx<T>*)h->ez;
+    a.ey = (const cplx<T>*)h->ey;
+    a.pz2 = h->pz2;
+    a.py2 = h->py2;
+    a.pz = h->pz;
+    a.py = h->py;
+    a.pinv_scale = (p.n_atoms / dV) / (8.0 * (double)h->N);
+    a.cdd = p.cdd;
+    a.dtau = p.imag_time_dt;
+    a.cx = -0.5 / (p.m * p.dx * p.dx);
+    a.cy = -0.5 / (p.m * p.dy * p.dy);
+    a.cz = -0.5 / (p.m * p.dz * p.dz);
+    a.gq = (h->ggp11 - p.cdd / 3) * p.n_atoms / dV;
+    a.gmq = h->gamma * p.w_15 / pow(dV, 1.5);
+    a.hdt = p.real_time_dt / 2.0;
+    a.cv = 1.0;
+    a.pg = a.gq;
+    a.pgm = a.gmq;
+    return a;
+}
+
+template <typename T>
+static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
+    LineArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    const gpe_params& p = h->p;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.cp = h->cp;
+    a.ck = h->ck;
+    a.tiles_kz = (p.nz + 1 + h->cp - 1) / h->cp;
+    a.tiles_k = (p.nz + h->ck - 1) / h->ck;
+    a.ntile_p = poisson ? 2 * p.ny * a.tiles_kz : 0;
+    a.s1 = (cplx<T>*)h->s1;
+    a.tk = (cplx<T>*)h->tk;
+    a.tw = (const cplx<T>*)h->tw;
+    a.ex = (const cplx<T>*)h->ex;
+    a.kx2 = h->kx2;
+    a.ky2 = h->ky2;
+    a.kz2 = h->kz2;
+    a.px2 = h->px2;
+    a.px = h->px;
+    (void)kinetic;
+    return a;
+}
+
+template <typename T>
+static int launch_plane(gpe_handle* h, const PlaneArgs<T>& a) {
+    hipLaunchKernelGGL((k_plane<T, kPlaneThreads>), dim3(h->p.nx), dim3(kPlaneThreads), h->plane_lds, h->stream, a);
+    GPE_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int launch_line(gpe_handle* h, bool poisson, bool kinetic) {
+    LineArgs<T> a = line_args<T>(h, poisson, kinetic);
+    const int nblk = a.ntile_p + (kinetic ? h->p.ny * a.tiles_k : 0);
+    if (nblk == 0) return 0;
+    hipLaunchKernelGGL((k_xpass<T, kLineThreads>), dim3(nblk), dim3(kLineThreads), h->line_lds, h->stream, a);
+    GPE_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int materialize(gpe_handle* h) {
+    if (!h->norm_pending) return 0;
+    hipLaunchKernelGGL((k_scale<T>), dim3(1024), dim3(256), 0, h->stream, (cplx<T>*)h->store[h->cur], h->N,
+                       h->part[h->pcur], h->p.nx);
+    GPE_CHECK(hipGetLastError());
+    h->norm_pending = false;
+    return 0;
+}
+
+template <typename T>
+static int do_calc_norm(gpe_handle* h, double* xnorma) {
+    if (materialize<T>(h)) return 1;
+    hipLaunchKernelGGL((k_norm_planes<T, 256>), dim3(h->p.nx), dim3(256), 0, h->stream,
+                       (const cplx<T>*)h->store[h->cur], h->p.ny * h->p.nz, h->part[h->pcur]);
+    GPE_CHECK(hipGetLastError());
+    std::vector<double> part(h->p.nx);
+    GPE_CHECK(hipMemcpyAsync(part.data(), h->part[h->pcur], h->p.nx * 8, hipMemcpyDeviceToHost, h->stream));
+    GPE_CHECK(hipStreamSynchronize(h->stream));
+    double s = 0;
+    for (double v : part) s += v;
+    if (xnorma) *xnorma = s;  // = dV * sum |psi|^2
+    return 0;
+}
+
+template <typename T>
+static int do_normalize(gpe_handle* h) {
+    // normalize() divides by sqrt of the norm computed by the last calc_norm
+    // (part[pcur]); the IT loop leaves the partials of the new state there.
+    hipLaunchKernelGGL((k_scale<T>), dim3(1024), dim3(256), 0, h->stream, (cplx<T>*)h->store[h->cur], h->N,
+                       h->part[h->pcur], h->p.nx);
+    GPE_CHECK(hipGetLastError());
+    h->norm_pending = false;
+    return 0;
+}
+
+template <typename T>
+static int do_calc_fi3d(gpe_handle* h) {
+    if (materialize<T>(h)) return 1;
+    PlaneArgs<T> a = plane_args<T>(h, M_RHO_FWD);
+    a.in = (const cplx<T>*)h->store[h->cur];
+    if (launch_plane<T>(h, a)) return 1;
+    if (launch_line<T>(h, true, false)) return 1;
+    a = plane_args<T>(h, M_PINV | M_PHI_STORE);
+    return launch_plane<T>(h, a);
+}
+
+template <typename T>
+static int do_it_hook(gpe_handle* h) {
+    if (materialize<T>(h)) return 1;
+    const gpe_params& p = h->p;
+    const double dV = p.dx * p.dy * p.dz;
+    hipLaunchKernelGGL((k_it_stencil<T>), dim3(2048), dim3(256), 0, h->stream, (const cplx<T>*)h->store[h->cur],
+                       (cplx<T>*)h->store[1 - h->cur], (const cplx<T>*)h->psii, (const T*)h->V, (const T*)h->phi,
+                       p.nx, p.ny, p.nz, p.imag_time_dt, -0.5 / (p.m * p.dx * p.dx), -0.5 / (p.m * p.dy * p.dy),
+                       -0.5 / (p.m * p.dz * p.dz), p.cdd, (h->ggp11 - p.cdd / 3) * p.n_atoms / dV,
+                       h->gamma * p.w_15 / pow(dV, 1.5));
+    GPE_CHECK(hipGetLastError());
+    h->cur = 1 - h->cur;
+    return 0;
+}
+
+template <typename T>
+static int do_potential_half(gpe_handle* h) {
+    if (materialize<T>(h)) return 1;
+    const gpe_params& p = h->p;
+    const double dV = p.dx * p.dy * p.dz;
+    hipLaunchKernelGGL((k_potential_half<T>), dim3(2048), dim3(256), 0, h->stream, (cplx<T>*)h->store[h->cur],
+                       (const T*)h->V, (const T*)h->phi, p.nx, p.ny, p.nz, p.real_time_dt / 2.0, p.cdd,
+                       (h->ggp11 - p.cdd / 3) * p.n_atoms / dV, h->gamma * p.w_15 / pow(dV, 1.5));
+    GPE_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int do_kinetic(gpe_handle* h) {
+    if (materialize<T>(h)) return 1;
+    PlaneArgs<T> a = plane_args<T>(h, M_KIN);
+    a.in = (const cplx<T>*)h->store[h->cur];
+    if (launch_plane<T>(h, a)) return 1;
+    if (launch_line<T>(h, false, true)) return 1;
+    a = plane_args<T>(h, M_IN_TK | M_OUT);
+    a.out = (cplx<T>*)h->store[h->cur];
+    return launch_plane<T>(h, a);
+}
+
+static void ev_record(gpe_handle* h, size_t idx) {
+    if (!h->timing) return;
+    while (h->ev.size() <= idx) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        h->ev.push_back(e);
+    }
+    (void)hipEventRecord(h->ev[idx], h->stream);
+}
+
+static int finish_timing(gpe_handle* h, size_t nev) {
+    if (!h->timing || nev < 2) return 0;
+    GPE_CHECK(hipEventSynchronize(h->ev[nev - 1]));
+    // events: [start, (after line, after plane)*]
+    double tl = 0, tp = 0;
+    int nl = 0, np = 0;
+    for (size_t t = 1; t < nev; ++t) {
+        float ms = 0;
+        GPE_CHECK(hipEventElapsedTime(&ms, h->ev[t - 1], h->ev[t]));
+        if (t % 2 == 1) {
+            tl += ms;
+            ++nl;
+        } else {
+            tp += ms;
+            ++np;
+        }
+    }
+    float tot = 0;
+    GPE_CHECK(hipEventElapsedTime(&tot, h->ev[0], h->ev[nev - 1]));
+    h->t_line_us = nl ? tl * 1e3 / nl : 0;
+    h->t_plane_us = np ? tp * 1e3 / np : 0;
+    h->t_total_ms = tot;
+    return 0;
+}
+
+template <typename T>
+static int do_it_iters(gpe_handle* h, long n) {
+    if (n <= 0) return 0;
+    // prologue: partials of the current state + Poisson start of |store|^2
+    PlaneArgs<T> a = plane_args<T>(h, M_RHO_FWD | M_NORM_OUT);
+    a.in = (const cplx<T>*)h->store[h->cur];
+    a.part_out = h->part[h->pcur];
+    if (launch_plane<T>(h, a)) return 1;
+    size_t ne = 0;
+    ev_record(h, ne++);
+    for (long t = 0; t < n; ++t) {
+        if (launch_line<T>(h, true, false)) return 1;
+        ev_record(h, ne++);
+        int mode = M_PINV | M_NORM_IN | M_IT | M_NORM_OUT | M_RHO_FWD;
+        if (t == n - 1) mode = (mode & ~M_RHO_FWD) | M_PHI_STORE;
+        a = plane_args<T>(h, mode);
+        a.in = (const cplx<T>*)h->store[h->cur];
+        a.out = (cplx<T>*)h->store[1 - h->cur];
+        a.part_in = h->part[h->pcur];
+        a.part_out = h->part[1 - h->pcur];
+        if (launch_plane<T>(h, a)) return 1;
+        ev_record(h, ne++);
+        h->cur = 1 - h->cur;
+        h->pcur = 1 - h->pcur;
+    }
+    h->norm_pending = true;
+    return finish_timing(h, ne);
+}
+
+template <typename T>
+static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm) {
+    if (n <= 0) return 0;
+    if (materialize<T>(h)) return 1;
+    const gpe_params& p = h->p;
+    const double dV = p.dx * p.dy * p.dz;
+    auto gq = [&](long t) { return ((g ? g[t] : h->ggp11) - p.cdd / 3) * p.n_atoms / dV; };
+    auto gmq = [&](long t) { return (gm ? gm[t] : h->gamma) * p.w_15 / pow(dV, 1.5); };
+    // prologue: first half phase with the current fi3d, Poisson start, z/y kinetic
+    PlaneArgs<T> a = plane_args<T>(h, M_PHI_LOAD | M_PHASE_HALF | M_RHO_FWD | M_KIN);
+    a.in = (const cplx<T>*)h->store[h->cur];
+    a.pg = gq(0);
+    a.pgm = gmq(0);
+    if (launch_plane<T>(h, a)) return 1;
+    size_t ne = 0;
+    ev_record(h, ne++);
+    for (long t = 0; t < n; ++t) {
+        if (launch_line<T>(h, true, true)) return 1;
+        ev_record(h, ne++);
+        if (t < n - 1) {
+            a = plane_args<T>(h, M_PINV | M_IN_TK | M_PHASE_FULL | M_RHO_FWD | M_KIN);
+            a.cv = 2.0;
+            a.pg = gq(t) + gq(t + 1);
+            a.pgm = gmq(t) + gmq(t + 1);
+        } else {
+            a = plane_args<T>(h, M_PINV | M_PHI_STORE | M_IN_TK | M_PHASE_HALF | M_OUT);
+            a.out = (cplx<T>*)h->store[h->cur];
+            a.pg = gq(t);
+            a.pgm = gmq(t);
+        }
+        if (launch_plane<T>(h, a)) return 1;
+        ev_record(h, ne++);
+    }
+    if (g) h->ggp11 = g[n - 1];
+    if (gm) h->gamma = gm[n - 1];
+    return finish_timing(h, ne);
+}
+
+template <typename T>
+static int do_energy(gpe_handle* h, gpe_energies* out) {
+    if (materialize<T>(h)) return 1;
+    const gpe_params& p = h->p;
+    const double dV = p.dx * p.dy * p.dz;
+    const double gq = (h->ggp11 - p.cdd / 3) * p.n_atoms / dV, gmq = h->gamma * p.w_15 / pow(dV, 1.5);
+    auto run = [&](int pass, double mu) -> int {
+        hipLaunchKernelGGL((k_energy<T, 256>), dim3(p.nx), dim3(256), 0, h->stream, (const cplx<T>*)h->store[h->cur],
+                           (const T*)h->V, (const T*)h->phi, p.nx, p.ny, p.nz, 1.0 / (p.dx * p.dx),
+                           1.0 / (p.dy * p.dy), 1.0 / (p.dz * p.dz), p.m, p.cdd, gq, gmq, mu, pass, h->epart);
+        GPE_CHECK(hipGetLastError());
+        return 0;
+    };
+    GPE_CHECK(hipMemsetAsync(h->epart, 0, (size_t)p.nx * 8 * 8, h->stream));
+    if (run(0, 0.0)) return 1;
+    std::vector<double> ep((size_t)p.nx * 8);
+    GPE_CHECK(hipMemcpyAsync(ep.data(), h->epart, ep.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    GPE_CHECK(hipStreamSynchronize(h->stream));
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < p.nx; ++i)
+        for (int t = 0; t < 6; ++t) s[t] += ep[(size_t)i * 8 + t];
+    const double mu = s[5];
+    if (run(1, mu)) return 1;
+    GPE_CHECK(hipMemcpyAsync(ep.data(), h->epart, ep.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    GPE_CHECK(hipStreamSynchronize(h->stream));
+    double r2 = 0;
+    for (int i = 0; i < p.nx; ++i) r2 += ep[(size_t)i * 8 + 6];
+    const double N = p.n_atoms;
+    // phi = sqrt(dV) psi: dV*sum f(psi) = sum f(phi) for quadratic f.
+    out->e_kin = s[0] / (2 * p.m) * N;
+    out->e_pot = s[1] * N;
+    out->e_int = 0.5 * h->ggp11 * s[2] / dV * N * N;
+    out->e_ext = 0.5 * p.cdd * s[3] * N - p.cdd / 3. * s[2] / dV / 2. * N * N;
+    out->e_bmf = 2. / 5. * h->gamma * s[4] / pow(dV, 1.5) * pow(N, 2.5);
+    out->e_total = out->e_kin + out->e_pot + out->e_int + out->e_ext + out->e_bmf;
+    out->mu = mu;
+    out->residual = sqrt(r2);
+    return 0;
+}
+
+#define DISPATCH(h, fn, ...) ((h)->prec == GPE_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+const char* gpe_version(void) { return "gpesolver-amd 0.1 (gfx950)"; }
+
+const char* gpe_last_error(const gpe_handle* h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
+    if (!p || !out) return fail(nullptr, "null argument");
+    *out = nullptr;
+    if (precision != GPE_FP32 && precision != GPE_FP64) return fail(nullptr, "precision must be 32 or 64");
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(nullptr, "grid dimensions must be >= 3");
+    if (!(p->dx > 0 && p->dy > 0 && p->dz > 0 && p->m > 0 && p->n_atoms > 0))
+        return fail(nullptr, "spacings, mass and n_atoms must be positive");
+    const int maxe = precision == GPE_FP64 ? Lim<double>::maxe : Lim<float>::maxe;
+    const long pe = 2L * p->ny * (p->nz + 1);
+    if (pe > maxe || (long)p->ny * p->nz > kMaxPlanePts)
+        return fail(nullptr, "grid plane " + std::to_string(p->ny) + "x" + std::to_string(p->nz) +
+                                 " exceeds the single-block plane path at this precision");
+    gpe_handle* h = new gpe_handle();
+    h->p = *p;
+    h->prec = precision;
+    h->N = (size_t)p->nx * p->ny * p->nz;
+    h->ggp11 = p->ggp11;
+    h->gamma = p->gamma;
+    h->sqdv = sqrt(p->dx * p->dy * p->dz);
+    auto bad = [&](const std::string& m) {
+        g_create_err = m;
+        gpe_destroy(h);
+        return 1;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bad("stream create failed");
+    const size_t cs = csize(h), rs = rsize(h);
+    const size_t n = h->N;
+    const size_t ns1 = (size_t)2 * p->ny * p->nx * (p->nz + 1);
+    if (hipMalloc(&h->store[0], n * cs) || hipMalloc(&h->store[1], n * cs) || hipMalloc(&h->psii, n * cs) ||
+        hipMalloc(&h->V, n * rs) || hipMalloc(&h->phi, n * rs) || hipMalloc(&h->s1, ns1 * cs) ||
+        hipMalloc(&h->tk, n * cs) || hipMalloc(&h->part[0], p->nx * 8) || hipMalloc(&h->part[1], p->nx * 8) ||
+        hipMalloc(&h->epart, (size_t)p->nx * 8 * 8))
+        return bad("device allocation failed");
+    if (hipMemset(h->store[0], 0, n * cs) || hipMemset(h->store[1], 0, n * cs) || hipMemset(h->psii, 0, n * cs) ||
+        hipMemset(h->V, 0, n * rs) || hipMemset(h->phi, 0, n * rs) || hipMemset(h->part[0], 0, p->nx * 8) ||
+        hipMemset(h->part[1], 0, p->nx * 8))
+        return bad("device memset failed");
+    int rc = precision == GPE_FP64 ? setup_tables<double>(h) : setup_tables<float>(h);
+    if (rc) return bad(h->err);
+    // launch shapes: Poisson tiles of cp kz-columns, kinetic tiles of ck k
+    if (2L * p->nx > kLineMaxE) return bad("x length too large for the line pass");
+    h->cp = std::min(p->nz + 1, std::max(1, kLineMaxE / (2 * p->nx)));
+    h->cp = std::min(h->cp, 12);
+    h->ck = std::min(p->nz, std::max(1, kLineMaxE / p->nx));
+    h->ck = std::min(h->ck, 16);
+    h->plane_lds = (size_t)std::max(2L * p->ny * (p->nz + 1), (long)p->ny * p->nz) * cs;
+    h->line_lds = (size_t)std::max(2L * p->nx * h->cp, (long)p->nx * h->ck) * cs;
+    if (hipDeviceSynchronize() != hipSuccess) return bad("device synchronize failed after setup");
+    *out = h;
+    return 0;
+}
+
+void gpe_destroy(gpe_handle* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void* bufs[] = {h->store[0], h->store[1], h->psii, h->V,  h->phi, h->s1,      h->tk,      h->tw,   h->ex,
+                    h->ey,       h->ez,       h->kx2,  h->ky2, h->kz2, h->part[0], h->part[1], h->epart};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int gpe_load_buffer(gpe_handle* h, const double* psi) {
+    if (!h || !psi) return fail(h, "null argument");
+    const size_t n2 = 2 * h->N;
+    if (upload_real(h, h->store[h->cur], psi, n2, h->sqdv)) return 1;
+    if (upload_real(h, h->psii, psi, n2, h->sqdv)) return 1;
+    h->norm_pending = false;
+    // calc_norm + normalize (solver.cpp:118-125)
+    if (DISPATCH(h, do_calc_norm, h, nullptr)) return 1;
+    return DISPATCH(h, do_normalize, h);
+}
+
+int gpe_load_pote(gpe_handle* h, const double* V) {
+    if (!h || !V) return fail(h, "null argument");
+    return upload_real(h, h->V, V, h->N, 1.0);
+}
+
+int gpe_export_psi(gpe_handle* h, double* psi) {
+    if (!h || !psi) return fail(h, "null argument");
+    if (DISPATCH(h, materialize, h)) return 1;
+    return download_real(h, psi, h->store[h->cur], 2 * h->N, 1.0 / h->sqdv);
+}
+
+int gpe_export_fi3d(gpe_handle* h, double* fi3d) {
+    if (!h || !fi3d) return fail(h, "null argument");
+    return download_real(h, fi3d, h->phi, h->N, 1.0);
+}
+
+int gpe_import_fi3d(gpe_handle* h, const double* fi3d) {
+    if (!h || !fi3d) return fail(h, "null argument");
+    return upload_real(h, h->phi, fi3d, h->N, 1.0);
+}
+
+int gpe_set_coupling(gpe_handle* h, double ggp11, double gamma) {
+    if (!h) return fail(h, "null handle");
+    h->ggp11 = ggp11;
+    h->gamma = gamma;
+    return 0;
+}
+
+int gpe_calc_fi3d(gpe_handle* h) { return h ? DISPATCH(h, do_calc_fi3d, h) : 1; }
+int gpe_calc_norm(gpe_handle* h, double* x) { return h ? DISPATCH(h, do_calc_norm, h, x) : 1; }
+int gpe_normalize(gpe_handle* h) { return h ? DISPATCH(h, do_normalize, h) : 1; }
+int gpe_imag_iter_step(gpe_handle* h) { return h ? DISPATCH(h, do_it_hook, h) : 1; }
+int gpe_real_potential_half_step(gpe_handle* h) { return h ? DISPATCH(h, do_potential_half, h) : 1; }
+int gpe_real_kinetic_step(gpe_handle* h) { return h ? DISPATCH(h, do_kinetic, h) : 1; }
+int gpe_calc_energy(gpe_handle* h, gpe_energies* out) {
+    if (!h || !out) return fail(h, "null argument");
+    return DISPATCH(h, do_energy, h, out);
+}
+int gpe_imag_time_iters(gpe_handle* h, long n) { return h ? DISPATCH(h, do_it_iters, h, n) : 1; }
+int gpe_real_time_iters(gpe_handle* h, long n, const double* g, const double* gm) {
+    return h ? DISPATCH(h, do_rt_iters, h, n, g, gm) : 1;
+}
+int gpe_synchronize(gpe_handle* h) {
+    if (!h) return 1;
+    GPE_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+int gpe_set_timing(gpe_handle* h, int on) {
+    if (!h) return 1;
+    h->timing = on != 0;
+    return 0;
+}
+int gpe_get_timing(gpe_handle* h, double* plane_us, double* line_us, double* total_ms) {
+    if (!h) return 1;
+    if (plane_us) *plane_us = h->t_plane_us;
+    if (line_us) *line_us = h->t_line_us;
+    if (total_ms) *total_ms = h->t_total_ms;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+"""ctypes binding of libgpehip.so (the C ABI declared in include/gpehip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded
+this module raises, it never falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpehip.so")
+
+GPE_FP32 = 32
+GPE_FP64 = 64
+
+# Every symbol include/gpehip.h declares.
+EXPORTS = (
+    "gpe_create", "gpe_destroy", "gpe_last_error", "gpe_version",
+    "gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d",
+    "gpe_set_coupling", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
+    "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy",
+    "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_synchronize", "gpe_set_timing", "gpe_get_timing",
+)
+
+
+class GpeParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("m", C.c_double), ("n_atoms", C.c_double),
+        ("cdd", C.c_double), ("ggp11", C.c_double), ("gamma", C.c_double), ("w_15", C.c_double),
+        ("imag_time_dt", C.c_double), ("real_time_dt", C.c_double),
+    ]
+
+
+class GpeEnergies(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("e_kin", "e_pot", "e_int", "e_ext", "e_bmf", "e_total", "mu", "residual")]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GpeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libgpehip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GpeError(f"{path} is missing: run `python -m gpesolver_amd.build` (or __graft_entry__.build())")
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    L.gpe_create.argtypes = [C.POINTER(GpeParams), ip, C.POINTER(vp)]
+    L.gpe_create.restype = ip
+    L.gpe_destroy.argtypes = [vp]
+    L.gpe_destroy.restype = None
+    L.gpe_last_error.argtypes = [vp]
+    L.gpe_last_error.restype = C.c_char_p
+    L.gpe_version.restype = C.c_char_p
+    for name in ("gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d"):
+        getattr(L, name).argtypes = [vp, dp]
+        getattr(L, name).restype = ip
+    L.gpe_set_coupling.argtypes = [vp, C.c_double, C.c_double]
+    L.gpe_set_coupling.restype = ip
+    for name in ("gpe_calc_fi3d", "gpe_normalize", "gpe_imag_iter_step", "gpe_real_potential_half_step",
+                 "gpe_real_kinetic_step", "gpe_synchronize"):
+        getattr(L, name).argtypes = [vp]
+        getattr(L, name).restype = ip
+    L.gpe_calc_norm.argtypes = [vp, dp]
+    L.gpe_calc_norm.restype = ip
+    L.gpe_calc_energy.argtypes = [vp, C.POINTER(GpeEnergies)]
+    L.gpe_calc_energy.restype = ip
+    L.gpe_imag_time_iters.argtypes = [vp, C.c_long]
+    L.gpe_imag_time_iters.restype = ip
+    L.gpe_real_time_iters.argtypes = [vp, C.c_long, dp, dp]
+    L.gpe_real_time_iters.restype = ip
+    L.gpe_set_timing.argtypes = [vp, ip]
+    L.gpe_set_timing.restype = ip
+    L.gpe_get_timing.argtypes = [vp, dp, dp, dp]
+    L.gpe_get_timing.restype = ip
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Device:
+    """One solver instance on the GPU (a gpe_handle)."""
+
+    def __init__(self, params: GpeParams, precision: int = GPE_FP64):
+        self.L = load()
+        self.params = params
+        self.precision = precision
+        self.shape = (params.nx, params.ny, params.nz)
+        h = C.c_void_p()
+        rc = self.L.gpe_create(C.byref(params), precision, C.byref(h))
+        if rc != 0:
+            raise GpeError(self.L.gpe_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gpe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise GpeError(self.L.gpe_last_error(self.h).decode())
+
+    # data movement
+    def load_buffer(self, psi: np.ndarray):
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert a.shape == self.shape
+        self._check(self.L.gpe_load_buffer(self.h, _dp(a.view(np.float64))))
+
+    def load_pote(self, V: np.ndarray):
+        a = np.ascontiguousarray(V, dtype=np.float64)
+        assert a.shape == self.shape
+        self._check(self.L.gpe_load_pote(self.h, _dp(a)))
+
+    def export_psi(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.complex128)
+        self._check(self.L.gpe_export_psi(self.h, _dp(out.view(np.float64))))
+        return out
+
+    def export_fi3d(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self._check(self.L.gpe_export_fi3d(self.h, _dp(out)))
+        return out
+
+    def import_fi3d(self, phi: np.ndarray):
+        a = np.ascontiguousarray(phi, dtype=np.float64)
+        self._check(self.L.gpe_import_fi3d(self.h, _dp(a)))
+
+    def set_coupling(self, ggp11: float, gamma: float):
+        self._check(self.L.gpe_set_coupling(self.h, ggp11, gamma))
+
+    # hooks
+    def calc_fi3d(self):
+        self._check(self.L.gpe_calc_fi3d(self.h))
+
+    def calc_norm(self) -> float:
+        x = C.c_double()
+        self._check(self.L.gpe_calc_norm(self.h, C.byref(x)))
+        return x.value
+
+    def normalize(self):
+        self._check(self.L.gpe_normalize(self.h))
+
+    def imag_iter_step(self):
+        self._check(self.L.gpe_imag_iter_step(self.h))
+
+    def potential_half_step(self):
+        self._check(self.L.gpe_real_potential_half_step(self.h))
+
+    def kinetic_step(self):
+        self._check(self.L.gpe_real_kinetic_step(self.h))
+
+    def energy(self) -> dict:
+        e = GpeEnergies()
+        self._check(self.L.gpe_calc_energy(self.h, C.byref(e)))
+        return e.as_dict()
+
+    # loops
+    def it_steps(self, n: int):
+        self._check(self.L.gpe_imag_time_iters(self.h, n))
+
+    def rt_steps(self, n: int, ggp11=None, gamma=None):
+        if ggp11 is None:
+            self._check(self.L.gpe_real_time_iters(self.h, n, None, None))
+        else:
+            g = np.ascontiguousarray(ggp11, dtype=np.float64)
+            gm = np.ascontiguousarray(gamma, dtype=np.float64)
+            assert g.shape == (n,) and gm.shape == (n,)
+            self._check(self.L.gpe_real_time_iters(self.h, n, _dp(g), _dp(gm)))
+
+    def synchronize(self):
+        self._check(self.L.gpe_synchronize(self.h))
+
+    def set_timing(self, on: bool):
+        self._check(self.L.gpe_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.L.gpe_get_timing(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"plane_us": a.value, "line_us": b.value, "total_ms": c.value}

@@ -1,0 +1,107 @@
+/*
+ * gpehip.h -- C ABI of the MI355X (gfx950) Gross-Pitaevskii solver backend.
+ *
+ * One handle = one solver instance on one HIP stream: the state of
+ * GpuGrossPitaevskiSolver / CpuGrossPitaevskiSolver in the reference
+ * (src/solver/cuda_solver/gpu_solver.hpp, src/solver/cpu_solver/cpu_solver.hpp),
+ * with the device buffers owned by the library and every host buffer owned by
+ * the caller.  No C++ exception crosses this boundary: every call returns 0 on
+ * success and a non-zero status otherwise; gpe_last_error() gives the message.
+ *
+ * Host arrays use the reference's StdMat3D layout
+ * (src/utils/types/mat3d/stdmat3d.hpp:26-28): row-major, index (i*ny+j)*nz+k,
+ * complex values interleaved (re, im) float64.
+ *
+ * Each entry point names the reference hook it replaces (file:line).
+ */
+#ifndef GPEHIP_H
+#define GPEHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpe_handle gpe_handle;
+
+/* Arithmetic of the bulk arrays.  Norm, energy and chemical-potential
+ * reductions are FP64 in both. */
+enum { GPE_FP32 = 32, GPE_FP64 = 64 };
+
+/* Solver constants in atomic units, computed on the host exactly as
+ * PhysicalParameters::init_parameters does (src/parameters/parameters.cpp:37-51),
+ * plus the two time steps (src/parameters/parameters.hpp:70-72). */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    double m;
+    double n_atoms;
+    double cdd;
+    double ggp11;
+    double gamma;
+    double w_15;
+    double imag_time_dt;
+    double real_time_dt;
+} gpe_params;
+
+/* Energy record: BECEnergies (src/context/context.hpp:16-27) plus the two
+ * quantities the convergence test uses.  Totals in hartree. */
+typedef struct {
+    double e_kin, e_pot, e_int, e_ext, e_bmf, e_total;
+    double mu;       /* <psi|H|psi>, H = the imaginary-time operator */
+    double residual; /* sqrt(dV * sum |H psi - mu psi|^2) over the interior */
+} gpe_energies;
+
+/* Create / destroy.  Replaces GpuGrossPitaevskiSolver::GpuGrossPitaevskiSolver +
+ * init_containers + prepare_fft (src/solver/cuda_solver/gpu_solver.cpp:4-38,
+ * src/solver/solver.cpp:15-23). */
+int gpe_create(const gpe_params *p, int precision, gpe_handle **out);
+void gpe_destroy(gpe_handle *h);
+/* Message of the last failed call on h (h == NULL: last failed gpe_create). */
+const char *gpe_last_error(const gpe_handle *h);
+/* Library version string. */
+const char *gpe_version(void);
+
+/* AbstractGrossPitaevskiSolver::load_buffer (src/solver/solver.cpp:118-125):
+ * cpsi = cpsii = psi, then calc_norm + normalize. */
+int gpe_load_buffer(gpe_handle *h, const double *psi);
+/* AbstractGrossPitaevskiSolver::load_pote (src/solver/solver.cpp:127-131). */
+int gpe_load_pote(gpe_handle *h, const double *V);
+/* export_data (src/solver/cuda_solver/gpu_solver.cpp:136-139): normalised cpsi. */
+int gpe_export_psi(gpe_handle *h, double *psi);
+/* fi3d (the dipolar potential buffer) to / from the host. */
+int gpe_export_fi3d(gpe_handle *h, double *fi3d);
+int gpe_import_fi3d(gpe_handle *h, const double *fi3d);
+/* PhysicalParameters::update_edd result (src/parameters/parameters.cpp:53-59):
+ * new contact and LHY couplings for the following steps. */
+int gpe_set_coupling(gpe_handle *h, double ggp11, double gamma);
+
+/* Hooks (src/solver/solver.hpp:55-63). */
+int gpe_calc_fi3d(gpe_handle *h);                   /* calc_fi3d: fftw_poisson_solver.cpp:51-96 */
+int gpe_calc_norm(gpe_handle *h, double *xnorma);   /* calc_norm: cpu_solver.cpp:19-34 */
+int gpe_normalize(gpe_handle *h);                   /* normalize: cpu_solver.cpp:36-48 */
+int gpe_imag_iter_step(gpe_handle *h);              /* imag_iter_linear_step + imag_iter_nonlinear_step:
+                                                       cpu_solver.cpp:50-97 */
+int gpe_real_potential_half_step(gpe_handle *h);    /* cpu_solver.cpp:99-131 */
+int gpe_real_kinetic_step(gpe_handle *h);           /* fftw_split_solver.cpp:44-86 */
+int gpe_calc_energy(gpe_handle *h, gpe_energies *out); /* cpu_solver.cpp:137-194 (+ mu, residual) */
+
+/* Fused loops: n consecutive imag_time_iter (src/solver/solver.cpp:99-105) /
+ * real_time_iter (src/solver/solver.cpp:107-112) calls.  For the real-time
+ * loop, ggp11/gamma may give one coupling pair per step (the update_edd ramp
+ * of src/solver/solver.cpp:76-77); NULL keeps the current coupling. */
+int gpe_imag_time_iters(gpe_handle *h, long n);
+int gpe_real_time_iters(gpe_handle *h, long n, const double *ggp11, const double *gamma);
+
+/* Wait for all work of h. */
+int gpe_synchronize(gpe_handle *h);
+
+/* Timing of the fused loops on the handle's stream: average duration per
+ * launch (microseconds) of the line pass and the plane pass over the last
+ * gpe_*_iters call when profiling is on.  Used by the benchmark. */
+int gpe_set_timing(gpe_handle *h, int on);
+int gpe_get_timing(gpe_handle *h, double *plane_us, double *line_us, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,257 @@
+"""GPU parity: the HIP backend (through the C ABI) against the FP64 CPU oracle
+on identical seeded inputs.
+
+Tolerances (relative L2 unless stated), from the solver's precision contract:
+  FP64 path: psi <= 1e-10, Phi <= 1e-12, energies <= 1e-10 relative
+  FP32 path: psi <= 1e-5 (IT), <= 1e-6 per RT step, Phi <= 1e-5, E/N <= 1e-6 relative
+"""
+import numpy as np
+import pytest
+
+from gpesolver_amd import hip
+from gpesolver_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 8), (24, 12, 6), (32, 24, 12)]
+TOL_PSI = {hip.GPE_FP64: 1e-10, hip.GPE_FP32: 1e-5}
+TOL_PHI = {hip.GPE_FP64: 1e-12, hip.GPE_FP32: 1e-5}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make_case(oracle, shape, pot="MEXICAN", init="MULTIPLE_GAUSS", seed=1, noise=1e-3, n_atoms=2e4, edd=1.5,
+              d=(120.0, 120.0, 600.0), dd_nm=None):
+    nx, ny, nz = shape
+    # keep the double well inside small boxes
+    dd_nm = dd_nm if dd_nm is not None else 0.2 * nx * d[0]
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, *d, n_atoms=n_atoms, edd=edd, dd_nm=dd_nm)
+    op = oracle.make_params(nx, ny, nz, *d, n_atoms=n_atoms, edd=edd, dd_nm=dd_nm)
+    V = P.potential(pot, p)
+    psi = P.wavefunction(init, p, n_gauss_max=2)
+    psi = P.add_phase_noise(psi, noise, seed)
+    return p, op, V, psi
+
+
+def make_pair(oracle, shape, prec, **kw):
+    p, op, V, psi = make_case(oracle, shape, **kw)
+    dev = hip.Device(p.to_gpe(), prec)
+    dev.load_pote(V)
+    dev.load_buffer(psi)
+    orc = oracle.Oracle(op)
+    orc.load_pote(V)
+    orc.load_buffer(psi)
+    return p, dev, orc
+
+
+@pytest.fixture(params=[hip.GPE_FP64, hip.GPE_FP32], ids=["fp64", "fp32"])
+def prec(request):
+    return request.param
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_load_buffer_normalises(oracle, shape, prec):
+    p, dev, orc = make_pair(oracle, shape, prec)
+    got = dev.export_psi()
+    assert rel(got, orc.psi) < TOL_PSI[prec]
+    assert abs(dev.calc_norm() - 1.0) < (1e-13 if prec == hip.GPE_FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_calc_fi3d(oracle, shape, prec):
+    p, dev, orc = make_pair(oracle, shape, prec)
+    dev.calc_fi3d()
+    orc.calc_fi3d()
+    assert rel(dev.export_fi3d(), orc.fi3d) < TOL_PHI[prec]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_it_hook_sequence(oracle, shape, prec):
+    """imag_time_iter through the individual hooks (solver.cpp:99-105)."""
+    p, dev, orc = make_pair(oracle, shape, prec)
+    for _ in range(3):
+        dev.calc_fi3d()
+        dev.imag_iter_step()
+        dev.calc_norm()
+        dev.normalize()
+        orc.it_step()
+    assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+    assert rel(dev.export_fi3d(), orc.fi3d) < TOL_PHI[prec]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_it_fused_steps(oracle, shape, prec):
+    # Single well: imaginary time is contractive here.  (In a small double
+    # well the left/right imbalance is an unstable direction of the explicit
+    # IT map, growing ~4x per step, so rounding differences between any two
+    # implementations are amplified until one well is chosen.)
+    p, dev, orc = make_pair(oracle, shape, prec, pot="HARMONIC", init="GAUSS")
+    dev.it_steps(25)
+    orc.it_steps(25)
+    assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+    assert rel(dev.export_fi3d(), orc.fi3d) < 10 * TOL_PHI[prec]
+    e, eo = dev.energy(), orc.energy()
+    tol = 1e-10 if prec == hip.GPE_FP64 else 1e-6
+    assert abs(e["e_total"] - eo["e_total"]) < tol * abs(eo["e_total"])
+    assert abs(e["mu"] - eo["mu"]) < tol * abs(eo["mu"])
+
+
+def test_it_fused_chunks_equal_one_call(oracle):
+    """Splitting the fused loop into chunks gives the same state (FP64)."""
+    p, dev, orc = make_pair(oracle, (24, 12, 6), hip.GPE_FP64, pot="HARMONIC", init="GAUSS")
+    p2, dev2, _ = make_pair(oracle, (24, 12, 6), hip.GPE_FP64, pot="HARMONIC", init="GAUSS")
+    dev.it_steps(12)
+    for _ in range(3):
+        dev2.it_steps(4)
+    assert rel(dev.export_psi(), dev2.export_psi()) < 1e-14
+
+
+def test_it_boundary_from_loaded_state(oracle):
+    """A loaded state with a non-zero boundary: every IT step resets the
+    boundary to the loaded (un-normalised) cpsii values before normalising
+    (cpu_solver.cpp:96, solver.cpp:118-125)."""
+    shape = (16, 12, 8)
+    p, op, V, psi = make_case(oracle, shape, pot="HARMONIC", init="GAUSS")
+    rng = np.random.default_rng(4)
+    psi = psi + 0.05 * np.abs(psi).max() * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    for prec in (hip.GPE_FP64, hip.GPE_FP32):
+        dev = hip.Device(p.to_gpe(), prec)
+        dev.load_pote(V)
+        dev.load_buffer(psi)
+        orc = oracle.Oracle(op)
+        orc.load_pote(V)
+        orc.load_buffer(psi)
+        dev.it_steps(7)
+        orc.it_steps(7)
+        assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_kinetic_step(oracle, shape, prec):
+    p, dev, orc = make_pair(oracle, shape, prec)
+    dev.kinetic_step()
+    orc.kinetic_step()
+    assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_potential_half_step(oracle, shape, prec):
+    p, dev, orc = make_pair(oracle, shape, prec)
+    dev.calc_fi3d()
+    orc.calc_fi3d()
+    dev.potential_half_step()
+    orc.potential_half_step()
+    assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_rt_fused_steps(oracle, shape, prec):
+    p, dev, orc = make_pair(oracle, shape, prec)
+    # start RT from a Phi computed on the loaded state (FS hand-over)
+    orc.calc_fi3d()
+    dev.import_fi3d(orc.fi3d.copy())
+    dev.rt_steps(1)
+    orc.rt_steps(0, 1)
+    # one step: FP32 error is the FFT round-off floor (~2-5e-7)
+    assert rel(dev.export_psi(), orc.psi) < (TOL_PSI[prec] if prec == hip.GPE_FP64 else 1e-6)
+    dev.rt_steps(19)
+    orc.rt_steps(1, 19)
+    # 20 steps: in these small, strongly interacting double wells the real-time
+    # dynamics amplify any perturbation ~20x over 20 steps (the FP64 path shows
+    # the same growth, 5e-15 -> 1e-13), so the FP32 bound is 3e-4 here.
+    tol = TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-4
+    assert rel(dev.export_psi(), orc.psi) < tol
+    assert rel(dev.export_fi3d(), orc.fi3d) < (10 * TOL_PHI[prec] if prec == hip.GPE_FP64 else 3e-4)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_rt_fused_single_well_fp32(oracle, shape):
+    """Single well (HARMONIC, GAUSS): FP32 real time stays within 2e-5 over 20 steps."""
+    p, dev, orc = make_pair(oracle, shape, hip.GPE_FP32, pot="HARMONIC", init="GAUSS")
+    orc.calc_fi3d()
+    dev.import_fi3d(orc.fi3d.copy())
+    dev.rt_steps(20)
+    orc.rt_steps(0, 20)
+    assert rel(dev.export_psi(), orc.psi) < 2e-5
+
+
+def test_rt_from_zero_fi3d(oracle):
+    """calc_strategy RT: the first half step sees fi3d = 0 (fresh buffers)."""
+    p, dev, orc = make_pair(oracle, (16, 16, 8), hip.GPE_FP64)
+    dev.rt_steps(3)
+    orc.rt_steps(0, 3)
+    assert rel(dev.export_psi(), orc.psi) < 1e-10
+
+
+def test_rt_edd_ramp(oracle, prec):
+    """update_edd before every step (solver.cpp:76-77, parameters.cpp:53-59)."""
+    shape = (24, 12, 6)
+    nx, ny, nz = shape
+    d = (120.0, 120.0, 600.0)
+    n = 12
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, *d, edd=None, edd_start=1.5, edd_stop=1.3, iter_real=n,
+                                           dd_nm=0.2 * nx * d[0])
+    op = oracle.make_params(nx, ny, nz, *d, edd=None, edd_start=1.5, edd_stop=1.3, iter_real=n,
+                            dd_nm=0.2 * nx * d[0])
+    V = P.potential("MEXICAN", p)
+    psi = P.add_phase_noise(P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2), 1e-3, 3)
+    dev = hip.Device(p.to_gpe(), prec)
+    dev.load_pote(V)
+    dev.load_buffer(psi)
+    orc = oracle.Oracle(op)
+    orc.load_pote(V)
+    orc.load_buffer(psi)
+    g = np.array([p.coupling_at(t)[1] for t in range(n)])
+    gm = np.array([p.coupling_at(t)[2] for t in range(n)])
+    dev.rt_steps(n, g, gm)
+    orc.rt_steps(0, n)
+    tol = TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-4  # see test_rt_fused_steps
+    assert rel(dev.export_psi(), orc.psi) < tol
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_energies(oracle, shape, prec):
+    p, dev, orc = make_pair(oracle, shape, prec)
+    dev.calc_fi3d()
+    orc.calc_fi3d()
+    e, eo = dev.energy(), orc.energy()
+    tol = 1e-10 if prec == hip.GPE_FP64 else 1e-6
+    for k in ("e_kin", "e_pot", "e_int", "e_ext", "e_bmf", "e_total", "mu"):
+        assert abs(e[k] - eo[k]) <= tol * max(abs(eo[k]), 1e-30) + 1e-30, (k, e[k], eo[k])
+    assert abs(e["residual"] - eo["residual"]) <= (1e-8 if prec == hip.GPE_FP64 else 1e-3) * eo["residual"]
+
+
+def test_config1_it_64x64x16(oracle):
+    """64x64x16, HARMONIC, GAUSS sigma = 4 cells, 200 IT steps (FP64 and FP32)."""
+    shape = (64, 64, 16)
+    p, op, V, psi = make_case(oracle, shape, pot="HARMONIC", init="GAUSS", noise=0.0)
+    psi = P.wavefunction("GAUSS", p, sigma_cells=4.0)
+    orc = oracle.Oracle(op)
+    orc.load_pote(V)
+    orc.load_buffer(psi)
+    orc.it_steps(200)
+    eo = orc.energy()
+    for prec in (hip.GPE_FP64, hip.GPE_FP32):
+        dev = hip.Device(p.to_gpe(), prec)
+        dev.load_pote(V)
+        dev.load_buffer(psi)
+        dev.it_steps(200)
+        e = dev.energy()
+        assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+        tol = 1e-10 if prec == hip.GPE_FP64 else 1e-6
+        assert abs(e["e_total"] - eo["e_total"]) < tol * abs(eo["e_total"])
+        dev.close()
+
+
+def test_deterministic_rerun(oracle):
+    """Deterministic reductions: the same run twice is bitwise identical."""
+    outs = []
+    for _ in range(2):
+        p, dev, orc = make_pair(oracle, (32, 24, 12), hip.GPE_FP32)
+        dev.it_steps(10)
+        dev.rt_steps(5)
+        outs.append((dev.export_psi(), dev.energy()["e_total"]))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
