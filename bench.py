@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark: real-time split-step throughput at 128x128x32 on one MI355X.
+
+Workload (one "step" = one real_time_iter of the reference,
+src/solver/solver.cpp:107-112): grid 128x128x32, dx=dy=120 nm, dz=600 nm,
+m=163.929 Da, n_atoms=2e4, eps_dd=1.5, MEXICAN potential with dd=1500 nm,
+MULTIPLE_GAUSS(2) initial state with SplitMix64 phase noise (1e-3 rad, seed
+20261203), dt = 1e-4/omega_x.  FP32 bulk arithmetic, FP64 reductions.  The
+field, potential and all tables are resident in HBM before the timed region.
+
+Prints ONE JSON line.  Besides steps/s it reports:
+  roofline      the dominant kernel's achieved bandwidth (its own bytes per
+                launch / its average launch time measured with HIP events on
+                the solver's stream) against the 8 TB/s HBM peak;
+  step_roofline the model bytes of one RT step (112N + 20H, FP32) x steps/s
+                against the same peak;
+  cpu_baseline  the FP64 CPU oracle (the project's restatement of
+                cpu_solver + FFTW solvers) timed on this host's cores on a
+                bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+GRID = (128, 128, 32)
+
+
+def build_case(precision):
+    from gpesolver_amd import hip
+    from gpesolver_amd import params as P
+
+    nx, ny, nz = GRID
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, 120.0, 120.0, 600.0, m_da=163.929, n_atoms=2e4, edd=1.5,
+                                           omega_hz=(60.0, 60.0, 120.0), dd_nm=1500.0)
+    p.real_time_dt = 1e-4 / p.omega_x
+    V = P.potential("MEXICAN", p)
+    psi = P.add_phase_noise(P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2), 1e-3, 20261203)
+    dev = hip.Device(p.to_gpe(), precision)
+    dev.load_pote(V)
+    dev.load_buffer(psi)
+    dev.calc_fi3d()
+    return p, dev, V, psi
+
+
+def model_bytes(nx, ny, nz, fp64=False):
+    """Algorithmic bytes of one RT step: 112N + 20H (FP32), H = 2nx*2ny*(nz+1)."""
+    N = nx * ny * nz
+    H = 2 * nx * 2 * ny * (nz + 1)
+    b = 112 * N + 20 * H
+    return 2 * b if fp64 else b
+
+
+def kernel_bytes(nx, ny, nz, cs=8, rs=4):
+    """Bytes each steady-state launch must move (its own inputs + outputs):
+    plane pass: read S1 + Tk + V, write S1 + Tk;  line pass: read+write S1 + Tk."""
+    N = nx * ny * nz
+    s1 = cs * 2 * ny * nx * (nz + 1)
+    tk = cs * N
+    return {"plane": 2 * s1 + 2 * tk + rs * N, "line": 2 * s1 + 2 * tk}
+
+
+def cpu_baseline(threads: int, budget_s: float = 12.0):
+    """FP64 oracle RT steps at the same grid, on `threads` host threads."""
+    from gpesolver_amd import params as P
+    from oracle import pyoracle
+
+    nx, ny, nz = GRID
+    pyoracle.set_threads(threads)
+    op = pyoracle.make_params(nx, ny, nz, 120.0, 120.0, 600.0, n_atoms=2e4, edd=1.5, dd_nm=1500.0)
+    op.real_time_dt = 1e-4 / op.omega_x
+    o = pyoracle.Oracle(op)
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, 120.0, 120.0, 600.0, n_atoms=2e4, edd=1.5, dd_nm=1500.0)
+    o.load_pote(P.potential("MEXICAN", p))
+    o.load_buffer(P.add_phase_noise(P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2), 1e-3, 20261203))
+    o.calc_fi3d()
+    o.rt_step()  # warm-up (plans, first touch)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.rt_step()
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 200:
+            break
+    return {"value": n / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 RT steps of the same 128x128x32 workload ({el:.1f} s), FFT threaded with OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--fp64", action="store_true", help="also time the FP64 path")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs one solver instance on one GPU; --gpus must be 1 "
+              "(multi-GPU slab decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from gpesolver_amd import hip
+
+    nx, ny, nz = GRID
+    p, dev, V, psi = build_case(hip.GPE_FP32)
+    # warm-up
+    dev.rt_steps(args.warmup)
+    dev.synchronize()
+    # timed region: exactly K steps
+    t0 = time.perf_counter()
+    dev.rt_steps(args.steps)
+    dev.synchronize()
+    el = time.perf_counter() - t0
+    steps_per_s = args.steps / el
+
+    # per-kernel durations from HIP events on the solver stream (separate pass)
+    dev.set_timing(True)
+    dev.rt_steps(min(args.steps, 1000))
+    tm = dev.timing()
+    dev.set_timing(False)
+    kb = kernel_bytes(nx, ny, nz)
+    kern = {"plane": tm["plane_us"], "line": tm["line_us"]}
+    dom = max(kern, key=kern.get)
+    achieved = kb[dom] / (kern[dom] * 1e-6)
+
+    # IT throughput on the same grid (FP32), fused loop
+    dev.it_steps(args.warmup)
+    dev.synchronize()
+    nit = max(100, args.steps // 3)
+    t1 = time.perf_counter()
+    dev.it_steps(nit)
+    dev.synchronize()
+    it_el = time.perf_counter() - t1
+    e = dev.energy()
+    dev.close()
+
+    extra = {}
+    if args.fp64:
+        _, d64, _, _ = build_case(hip.GPE_FP64)
+        d64.rt_steps(args.warmup)
+        d64.synchronize()
+        t2 = time.perf_counter()
+        d64.rt_steps(args.steps)
+        d64.synchronize()
+        extra["fp64_rt_steps_per_s"] = args.steps / (time.perf_counter() - t2)
+        d64.close()
+
+    mb = model_bytes(nx, ny, nz)
+    out = {
+        "metric": "RT split-step steps/s @128²×32 (%HBM roof); IT time-to-ΔE<1e-6, warm vs plain",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "RT split-step 128x128x32, MEXICAN dd=1500nm, MULTIPLE_GAUSS(2)+phase noise, "
+                        "dt=1e-4/omega_x, FP32 bulk / FP64 reductions",
+            "grid": list(GRID),
+            "spacing_nm": [120.0, 120.0, 600.0],
+            "n_atoms": 2e4,
+            "edd": 1.5,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_{'plane' if dom == 'plane' else 'xpass'} (RT steady state)",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "bytes_per_launch": kb[dom],
+            "avg_launch_us": kern[dom],
+            "other_kernel_us": {k: v for k, v in kern.items() if k != dom},
+        },
+        "step_roofline": {
+            "model_bytes_per_step": mb,
+            "achieved_GBps": mb * steps_per_s / 1e9,
+            "frac": mb * steps_per_s / HBM_PEAK,
+        },
+        "it_steps_per_s": nit / it_el,
+        "it_energy_per_atom": e["e_total"] / p.n_atoms,
+        **extra,
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
