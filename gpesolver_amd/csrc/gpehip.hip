@@ -34,6 +34,7 @@
 
 #include "../../include/gpehip.h"
 #include "gpe_fft.hpp"
+#include "gpe_fft_ct.hpp"
 
 using gpe::cplx;
 using gpe::LinePlan;
@@ -159,17 +160,103 @@ __device__ double sum_partials(const double* p, int n, double* red) {
 }
 
 // ---------------------------------------------------------------------------
+// Size policies: the kernels are written once; a policy supplies the grid
+// sizes and the line FFTs, either at run time (any 2^a 3^b grid, generic
+// radix-4/2/3 engine) or as compile-time constants (large-radix engine).
+struct PlanePolGeneric {
+    template <class A>
+    __device__ static int ny(const A& a) { return a.ny; }
+    template <class A>
+    __device__ static int nz(const A& a) { return a.nz; }
+    template <int NT>
+    static constexpr int ppt() { return kMaxPlanePts / NT; }
+    template <typename T, int NT>
+    static constexpr int rq() { return (Lim<T>::maxe / 2 + NT - 1) / NT; }
+    template <typename T, int NT, int S>
+    __device__ static void fft_y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py2, nl, es, ls, S, a.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz2, nl, es, ls, S, a.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py, nl, es, ls, S, a.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz, nl, es, ls, S, a.tw);
+    }
+};
+
+template <int NY, int NZ>
+struct PlanePolCT {
+    template <class A>
+    __device__ static constexpr int ny(const A&) { return NY; }
+    template <class A>
+    __device__ static constexpr int nz(const A&) { return NZ; }
+    template <int NT>
+    static constexpr int ppt() { return (NY * NZ + NT - 1) / NT; }
+    template <typename T, int NT>
+    static constexpr int rq() { return (NY * (NZ + 1) + NT - 1) / NT; }
+    template <typename T, int NT, int S>
+    __device__ static void fft_y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, 2 * NY, NT, NZ + 1, S>(b, nl, es, ls, a.tw + a.py2.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, 2 * NZ, NT, NY, S>(b, nl, es, ls, a.tw + a.pz2.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, NY, NT, NZ, S>(b, nl, es, ls, a.tw + a.py.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, NZ, NT, NY, S>(b, nl, es, ls, a.tw + a.pz.tw);
+    }
+};
+
+struct LinePolGeneric {
+    template <class A>
+    __device__ static int nx(const A& a) { return a.nx; }
+    template <typename T, int NT, int S>
+    __device__ static void fft_x2(cplx<T>* b, int nl, int es, int ls, const LineArgs<T>& a) {
+        gpe::lds_fft<T, NT, kLineMaxE>(b, a.px2, nl, es, ls, S, a.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_x(cplx<T>* b, int nl, int es, int ls, const LineArgs<T>& a) {
+        gpe::lds_fft<T, NT, kLineMaxE>(b, a.px, nl, es, ls, S, a.tw);
+    }
+};
+
+template <int NX>
+struct LinePolCT {
+    template <class A>
+    __device__ static constexpr int nx(const A&) { return NX; }
+    template <typename T, int NT, int S>
+    __device__ static void fft_x2(cplx<T>* b, int nl, int es, int ls, const LineArgs<T>& a) {
+        gpe::ct::fft<T, 2 * NX, NT, kLineMaxE / (2 * NX), S>(b, nl, es, ls, a.tw + a.px2.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void fft_x(cplx<T>* b, int nl, int es, int ls, const LineArgs<T>& a) {
+        gpe::ct::fft<T, NX, NT, kLineMaxE / NX, S>(b, nl, es, ls, a.tw + a.px.tw);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Plane kernel.  Dynamic LDS: max(2ny*(nz+1), ny*nz) complex T.
-template <typename T, int NT>
+template <typename T, int NT, class Pol>
 __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
-    constexpr int MAXE = Lim<T>::maxe;
-    constexpr int PPT = kMaxPlanePts / NT;  // points per thread
+    constexpr int PPT = Pol::template ppt<NT>();  // points per thread
+    constexpr int RQ = Pol::template rq<T, NT>();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* P = reinterpret_cast<cplx<T>*>(lds_raw);
     __shared__ double red[NT / 64 + 1];
 
     const int i = blockIdx.x;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int nx = a.nx, ny = Pol::ny(a), nz = Pol::nz(a);
     const int nzh = nz + 1, nz2 = 2 * nz, ny2 = 2 * ny;
     const int npl = ny * nz;
     const int mode = a.mode;
@@ -189,9 +276,8 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
             P[e] = a.s1[((size_t)ky * nx + i) * nzh + kz];
         }
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(P, a.py2, nzh, nzh, 1, +1, a.tw);
+        Pol::template fft_y2<T, NT, +1>(P, nzh, nzh, 1, a);
         // rows j < ny -> Z[j][0..2nz) with Hermitian extension, via registers
-        constexpr int RQ = (MAXE / 2 + NT - 1) / NT;
         cplx<T> tmp[RQ];
 #pragma unroll
         for (int q = 0; q < RQ; ++q) {
@@ -209,7 +295,7 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
             }
         }
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(P, a.pz2, ny, 1, nz2, +1, a.tw);
+        Pol::template fft_z2<T, NT, +1>(P, ny, 1, nz2, a);
         const T sc = (T)(a.pinv_scale * s_in * s_in);
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
@@ -322,15 +408,15 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
         }
         if (mode & M_KIN) {
             // z propagation: rows j, length nz
-            gpe::lds_fft<T, NT, MAXE>(Q, a.pz, ny, 1, nz, -1, a.tw);
+            Pol::template fft_z<T, NT, -1>(Q, ny, 1, nz, a);
             for (int p = threadIdx.x; p < npl; p += NT) Q[p] = gpe::cmul(Q[p], a.ez[p % nz]);
             __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(Q, a.pz, ny, 1, nz, +1, a.tw);
+            Pol::template fft_z<T, NT, +1>(Q, ny, 1, nz, a);
             // y propagation: columns k, length ny
-            gpe::lds_fft<T, NT, MAXE>(Q, a.py, nz, nz, 1, -1, a.tw);
+            Pol::template fft_y<T, NT, -1>(Q, nz, nz, 1, a);
             for (int p = threadIdx.x; p < npl; p += NT) Q[p] = gpe::cmul(Q[p], a.ey[p / nz]);
             __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(Q, a.py, nz, nz, 1, +1, a.tw);
+            Pol::template fft_y<T, NT, +1>(Q, nz, nz, 1, a);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
                 a.tk[((size_t)j * nx + i) * nz + k] = Q[p];
@@ -357,8 +443,7 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
             }
         }
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(P, a.pz2, ny, 1, nz2, -1, a.tw);
-        constexpr int RQ = (MAXE / 2 + NT - 1) / NT;
+        Pol::template fft_z2<T, NT, -1>(P, ny, 1, nz2, a);
         cplx<T> tmp[RQ];
 #pragma unroll
         for (int q = 0; q < RQ; ++q) {
@@ -376,7 +461,7 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
         }
         for (int e = ny * nzh + threadIdx.x; e < ny2 * nzh; e += NT) P[e] = gpe::mk(T(0), T(0));
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(P, a.py2, nzh, nzh, 1, -1, a.tw);
+        Pol::template fft_y2<T, NT, -1>(P, nzh, nzh, 1, a);
         for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
             const int ky = e / nzh, kz = e - ky * nzh;
             a.s1[((size_t)ky * nx + i) * nzh + kz] = P[e];
@@ -386,12 +471,11 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
 
 // ---------------------------------------------------------------------------
 // Line kernel: x-direction passes.  Dynamic LDS: max(2nx*cp, nx*ck) complex T.
-template <typename T, int NT>
+template <typename T, int NT, class Pol>
 __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
-    constexpr int MAXE = kLineMaxE;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
-    const int nx = a.nx, nzh = a.nz + 1;
+    const int nx = Pol::nx(a), nzh = a.nz + 1;
     int b = blockIdx.x;
     if (b < a.ntile_p) {
         const int ky = b / a.tiles_kz;
@@ -404,7 +488,7 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
             X[e] = (ii < nx) ? base[(size_t)ii * nzh + c] : gpe::mk(T(0), T(0));
         }
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(X, a.px2, C, C, 1, -1, a.tw);
+        Pol::template fft_x2<T, NT, -1>(X, C, C, 1, a);
         const double ky2 = a.ky2[ky];
         for (int e = threadIdx.x; e < nx2 * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
@@ -414,7 +498,7 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
             X[e] = gpe::cscale(X[e], v);
         }
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(X, a.px2, C, C, 1, +1, a.tw);
+        Pol::template fft_x2<T, NT, +1>(X, C, C, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
             base[(size_t)ii * nzh + c] = X[e];
@@ -431,10 +515,10 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
             X[e] = base[(size_t)ii * nz + c];
         }
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(X, a.px, C, C, 1, -1, a.tw);
+        Pol::template fft_x<T, NT, -1>(X, C, C, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) X[e] = gpe::cmul(X[e], a.ex[e / C]);
         __syncthreads();
-        gpe::lds_fft<T, NT, MAXE>(X, a.px, C, C, 1, +1, a.tw);
+        Pol::template fft_x<T, NT, +1>(X, C, C, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
             base[(size_t)ii * nz + c] = X[e];
@@ -635,6 +719,7 @@ struct gpe_handle {
     double* epart = nullptr;
     int pcur = 0;
     bool norm_pending = false;  // true: normalised psi = store[cur]/sqrt(sum part[pcur])
+    bool force_generic = false; // GPE_FORCE_GENERIC=1: run-time-size kernels only
     LinePlan px, py, pz, px2, py2, pz2;
     // launch shapes
     int cp = 0, ck = 0;
@@ -842,9 +927,33 @@ static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
     return a;
 }
 
+template <typename T, class Pol>
+static int launch_plane_pol(gpe_handle* h, const PlaneArgs<T>& a) {
+    hipLaunchKernelGGL((k_plane<T, kPlaneThreads, Pol>), dim3(h->p.nx), dim3(kPlaneThreads), h->plane_lds, h->stream,
+                       a);
+    GPE_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Plane shapes (ny, nz) and x lengths with compile-time kernels.
+#define GPE_PLANE_SHAPES(X) X(128, 32) X(64, 16) X(96, 24) X(16, 8) X(24, 12) X(12, 6)
+#define GPE_LINE_SHAPES(X) X(128) X(64) X(96) X(16) X(24) X(32)
+
 template <typename T>
 static int launch_plane(gpe_handle* h, const PlaneArgs<T>& a) {
-    hipLaunchKernelGGL((k_plane<T, kPlaneThreads>), dim3(h->p.nx), dim3(kPlaneThreads), h->plane_lds, h->stream, a);
+    const int ny = h->p.ny, nz = h->p.nz;
+    if (!h->force_generic) {
+#define GPE_PL(NY, NZ) \
+    if (ny == NY && nz == NZ) return launch_plane_pol<T, PlanePolCT<NY, NZ>>(h, a);
+        GPE_PLANE_SHAPES(GPE_PL)
+#undef GPE_PL
+    }
+    return launch_plane_pol<T, PlanePolGeneric>(h, a);
+}
+
+template <typename T, class Pol>
+static int launch_line_pol(gpe_handle* h, const LineArgs<T>& a, int nblk) {
+    hipLaunchKernelGGL((k_xpass<T, kLineThreads, Pol>), dim3(nblk), dim3(kLineThreads), h->line_lds, h->stream, a);
     GPE_CHECK(hipGetLastError());
     return 0;
 }
@@ -854,9 +963,14 @@ static int launch_line(gpe_handle* h, bool poisson, bool kinetic) {
     LineArgs<T> a = line_args<T>(h, poisson, kinetic);
     const int nblk = a.ntile_p + (kinetic ? h->p.ny * a.tiles_k : 0);
     if (nblk == 0) return 0;
-    hipLaunchKernelGGL((k_xpass<T, kLineThreads>), dim3(nblk), dim3(kLineThreads), h->line_lds, h->stream, a);
-    GPE_CHECK(hipGetLastError());
-    return 0;
+    const int nx = h->p.nx;
+    if (!h->force_generic) {
+#define GPE_LN(NX) \
+    if (nx == NX) return launch_line_pol<T, LinePolCT<NX>>(h, a, nblk);
+        GPE_LINE_SHAPES(GPE_LN)
+#undef GPE_LN
+    }
+    return launch_line_pol<T, LinePolGeneric>(h, a, nblk);
 }
 
 template <typename T>
@@ -1114,6 +1228,10 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     h->ggp11 = p->ggp11;
     h->gamma = p->gamma;
     h->sqdv = sqrt(p->dx * p->dy * p->dz);
+    {
+        const char* fg = getenv("GPE_FORCE_GENERIC");
+        h->force_generic = fg && fg[0] == '1';
+    }
     auto bad = [&](const std::string& m) {
         g_create_err = m;
         gpe_destroy(h);

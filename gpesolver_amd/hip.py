@@ -24,6 +24,8 @@ EXPORTS = (
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_synchronize", "gpe_set_timing", "gpe_get_timing",
 )
+# Every symbol include/gpesolver_host.h declares.
+HOST_EXPORTS = ("gpes_validate_params_json", "gpes_run")
 
 
 class GpeParams(C.Structure):
@@ -88,6 +90,10 @@ def load(path: str = LIB_PATH):
     L.gpe_set_timing.restype = ip
     L.gpe_get_timing.argtypes = [vp, dp, dp, dp]
     L.gpe_get_timing.restype = ip
+    L.gpes_validate_params_json.argtypes = [C.c_char_p, C.c_char_p, ip]
+    L.gpes_validate_params_json.restype = ip
+    L.gpes_run.argtypes = [C.c_char_p, C.c_double, dp, C.POINTER(C.c_long), C.c_char_p, ip]
+    L.gpes_run.restype = ip
     _lib = L
     return L
 
@@ -203,3 +209,24 @@ class Device:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         self._check(self.L.gpe_get_timing(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"plane_us": a.value, "line_us": b.value, "total_ms": c.value}
+
+
+def validate_params_json(text: str) -> str | None:
+    """FileManager::load_all_v1 + check_params on a JSON text: None if valid,
+    else the error message (the reference throws std::runtime_error)."""
+    L = load()
+    buf = C.create_string_buffer(512)
+    rc = L.gpes_validate_params_json(text.encode(), buf, 512)
+    return None if rc == 0 else buf.value.decode()
+
+
+def run_directory(workdir: str, it_tol: float = 0.0) -> dict:
+    """Run the host front end (gpe_params.json in workdir) on the GPU."""
+    L = load()
+    e = C.c_double()
+    n = C.c_long()
+    buf = C.create_string_buffer(512)
+    rc = L.gpes_run(workdir.encode(), it_tol, C.byref(e), C.byref(n), buf, 512)
+    if rc != 0:
+        raise GpeError(buf.value.decode())
+    return {"energy_per_atom": e.value, "it_steps": n.value}

@@ -15,9 +15,9 @@ from gpesolver_amd import params as P
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "gpehip.h")).read()
-    return sorted(set(re.findall(r"\b(gpe_[a-z_0-9]+)\s*\(", txt)))
+def header_symbols(name="gpehip.h", prefix="gpe_"):
+    txt = open(os.path.join(ROOT, "include", name)).read()
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -25,6 +25,9 @@ def test_library_exports_every_declared_symbol():
     declared = header_symbols()
     assert declared, "no declarations found"
     assert set(declared) == set(hip.EXPORTS)
+    host = header_symbols("gpesolver_host.h", "gpes_")
+    assert set(host) == set(hip.HOST_EXPORTS)
+    declared = declared + host
     for name in declared:
         assert hasattr(lib, name), name
     assert b"gfx950" in lib.gpe_version()

@@ -255,3 +255,20 @@ def test_deterministic_rerun(oracle):
         outs.append((dev.export_psi(), dev.energy()["e_total"]))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert outs[0][1] == outs[1][1]
+
+
+@pytest.mark.parametrize("shape,force", [((36, 18, 12), "0"), ((32, 24, 12), "1"), ((16, 16, 8), "1")])
+def test_generic_runtime_size_kernels(oracle, shape, force, monkeypatch):
+    """Grids without compile-time kernels (36x18x12), and GPE_FORCE_GENERIC=1
+    on compiled shapes, run the run-time-size radix-4/2/3 engine."""
+    monkeypatch.setenv("GPE_FORCE_GENERIC", force)
+    for prec in (hip.GPE_FP64, hip.GPE_FP32):
+        p, dev, orc = make_pair(oracle, shape, prec, pot="HARMONIC", init="GAUSS")
+        dev.it_steps(10)
+        orc.it_steps(10)
+        assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+        orc.calc_fi3d()
+        dev.import_fi3d(orc.fi3d.copy())
+        dev.rt_steps(3)
+        orc.rt_steps(0, 3)
+        assert rel(dev.export_psi(), orc.psi) < (TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-6)
