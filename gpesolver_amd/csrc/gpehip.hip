@@ -46,6 +46,10 @@ constexpr int kLineThreads = 256;
 constexpr int kMaxPlanePts = 8192;  // ny*nz bound of the single-block plane path
 constexpr int kLineMaxE = 4096;     // complex elements of one x-line tile
 
+// Plane shapes (ny, nz) and x lengths with compile-time kernels.
+#define GPE_PLANE_SHAPES(X) X(128, 32) X(64, 16) X(96, 24) X(16, 8) X(24, 12) X(12, 6)
+#define GPE_LINE_SHAPES(X) X(128) X(64) X(96) X(16) X(24) X(32)
+
 template <typename T>
 struct Lim;
 template <>
@@ -76,6 +80,7 @@ enum : int {
 template <typename T>
 struct PlaneArgs {
     int nx, ny, nz, mode;
+    int generic;  // 1: run-time-size kernels even for a compiled shape
     const cplx<T>* in;
     cplx<T>* out;
     const cplx<T>* psii;
@@ -95,11 +100,17 @@ struct PlaneArgs {
     double dtau, cx, cy, cz, gq, gmq;
     // RT phase: theta = -hdt * (cv*(V + cdd*Phi) + pg*rho + pgm*rho^1.5)
     double hdt, cv, pg, pgm;
+    // Optional per-step coupling table (scaled g', gamma' pairs) indexed by
+    // *cbase + cidx; when set it overrides pg / pgm (the update_edd ramp).
+    const double* coef;
+    const int* cbase;
+    int cidx;
 };
 
 template <typename T>
 struct LineArgs {
     int nx, ny, nz;
+    int generic;
     int ntile_p;  // Poisson tiles (first blocks)
     int cp;       // kz per Poisson tile
     int ck;       // k per kinetic tile
@@ -248,14 +259,10 @@ struct LinePolCT {
 // ---------------------------------------------------------------------------
 // Plane kernel.  Dynamic LDS: max(2ny*(nz+1), ny*nz) complex T.
 template <typename T, int NT, class Pol>
-__global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
+__device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, unsigned char* lds_raw, double* red) {
     constexpr int PPT = Pol::template ppt<NT>();  // points per thread
     constexpr int RQ = Pol::template rq<T, NT>();
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* P = reinterpret_cast<cplx<T>*>(lds_raw);
-    __shared__ double red[NT / 64 + 1];
-
-    const int i = blockIdx.x;
     const int nx = a.nx, ny = Pol::ny(a), nz = Pol::nz(a);
     const int nzh = nz + 1, nz2 = 2 * nz, ny2 = 2 * ny;
     const int npl = ny * nz;
@@ -379,6 +386,16 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
         }
         __syncthreads();
         const bool iint = (i >= 1 && i <= nx - 2);
+        double pg = a.pg, pgm = a.pgm;
+        if (a.coef) {
+            const int t = (a.cbase ? *a.cbase : 0) + a.cidx;
+            pg = a.coef[2 * t];
+            pgm = a.coef[2 * t + 1];
+            if (mode & M_PHASE_FULL) {
+                pg += a.coef[2 * t + 2];
+                pgm += a.coef[2 * t + 3];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int p = threadIdx.x + q * NT;
@@ -389,8 +406,8 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
                     const int j = p / nz, k = p - j * nz;
                     if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
                         const T ab = sqrt(nrm);
-                        const T veff = (T)a.cv * (a.V[plane_off + p] + (T)a.cdd * phi_r[q]) + (T)a.pg * nrm +
-                                       (T)a.pgm * (ab * ab * ab);
+                        const T veff = (T)a.cv * (a.V[plane_off + p] + (T)a.cdd * phi_r[q]) + (T)pg * nrm +
+                                       (T)pgm * (ab * ab * ab);
                         T sn, cs;
                         sincos_t<T>(-(T)a.hdt * veff, &sn, &cs);
                         c = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
@@ -472,11 +489,9 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Line kernel: x-direction passes.  Dynamic LDS: max(2nx*cp, nx*ck) complex T.
 template <typename T, int NT, class Pol>
-__global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+__device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned char* lds_raw) {
     cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
     const int nx = Pol::nx(a), nzh = a.nz + 1;
-    int b = blockIdx.x;
     if (b < a.ntile_p) {
         const int ky = b / a.tiles_kz;
         const int kz0 = (b - ky * a.tiles_kz) * a.cp;
@@ -526,6 +541,58 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
     }
 }
 
+template <typename T, int NT, class Pol>
+__global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[NT / 64 + 1];
+    plane_body<T, NT, Pol>(a, blockIdx.x, lds_raw, red);
+}
+
+template <typename T, int NT, class Pol>
+__global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    line_body<T, NT, Pol>(a, blockIdx.x, lds_raw);
+}
+
+// Batched launches: one block per work item (instance, plane) / (instance,
+// tile); each instance brings its own argument block and may have its own grid.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_plane_mixed(const PlaneArgs<T>* __restrict__ args,
+                                                    const int2* __restrict__ work) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[NT / 64 + 1];
+    const int2 w = work[blockIdx.x];
+    const PlaneArgs<T> a = args[w.x];
+    if (!a.generic) {
+#define GPE_PL(NY, NZ)                                                  \
+    if (a.ny == NY && a.nz == NZ) {                                     \
+        plane_body<T, NT, PlanePolCT<NY, NZ>>(a, w.y, lds_raw, red);    \
+        return;                                                         \
+    }
+        GPE_PLANE_SHAPES(GPE_PL)
+#undef GPE_PL
+    }
+    plane_body<T, NT, PlanePolGeneric>(a, w.y, lds_raw, red);
+}
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_xpass_mixed(const LineArgs<T>* __restrict__ args,
+                                                    const int2* __restrict__ work) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int2 w = work[blockIdx.x];
+    const LineArgs<T> a = args[w.x];
+    if (!a.generic) {
+#define GPE_LN(NX)                                              \
+    if (a.nx == NX) {                                           \
+        line_body<T, NT, LinePolCT<NX>>(a, w.y, lds_raw);       \
+        return;                                                 \
+    }
+        GPE_LINE_SHAPES(GPE_LN)
+#undef GPE_LN
+    }
+    line_body<T, NT, LinePolGeneric>(a, w.y, lds_raw);
+}
+
 // ---------------------------------------------------------------------------
 // Elementwise / reduction kernels.
 
@@ -544,6 +611,11 @@ __global__ void k_scale(cplx<T>* psi, size_t n, const double* part, int np) {
         cplx<T> v = psi[e];
         psi[e] = gpe::mk(v.x * sc, v.y * sc);
     }
+}
+
+// Step counter of the graph-replayed loops (one thread).
+__global__ void k_advance(int* base, int inc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *base += inc;
 }
 
 // Per-plane sum |psi|^2 (calc_norm), one block per plane.
@@ -699,6 +771,7 @@ struct gpe_handle {
     int prec;
     size_t N;
     hipStream_t stream = nullptr;
+    bool own_stream = true;
     std::string err;
     // device buffers (bytes allocated per precision)
     void* store[2] = {nullptr, nullptr};
@@ -727,6 +800,14 @@ struct gpe_handle {
     // scaled constants
     double sqdv = 1.0;
     double ggp11, gamma;
+    // graph-replayed steady state (chunks of kGraphSteps steps)
+    double* coef = nullptr;      // per-step coupling table (2 doubles per step)
+    size_t coef_cap = 0;         // steps the table holds
+    int* cbase = nullptr;        // device step counter read by graph nodes
+    hipGraphExec_t rt_graph = nullptr;
+    hipGraphExec_t it_graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    double it_graph_key[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    bool use_graphs = true;      // GPE_NO_GRAPH=1 disables
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -874,6 +955,7 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.ny = p.ny;
     a.nz = p.nz;
     a.mode = mode;
+    a.generic = h->force_generic ? 1 : 0;
     a.psii = (const cplx<T>*)h->psii;
     a.tk = (cplx<T>*)h->tk;
     a.s1 = (cplx<T>*)h->s1;
@@ -909,6 +991,7 @@ static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
     a.nx = p.nx;
     a.ny = p.ny;
     a.nz = p.nz;
+    a.generic = h->force_generic ? 1 : 0;
     a.cp = h->cp;
     a.ck = h->ck;
     a.tiles_kz = (p.nz + 1 + h->cp - 1) / h->cp;
@@ -935,9 +1018,6 @@ static int launch_plane_pol(gpe_handle* h, const PlaneArgs<T>& a) {
     return 0;
 }
 
-// Plane shapes (ny, nz) and x lengths with compile-time kernels.
-#define GPE_PLANE_SHAPES(X) X(128, 32) X(64, 16) X(96, 24) X(16, 8) X(24, 12) X(12, 6)
-#define GPE_LINE_SHAPES(X) X(128) X(64) X(96) X(16) X(24) X(32)
 
 template <typename T>
 static int launch_plane(gpe_handle* h, const PlaneArgs<T>& a) {
@@ -1094,6 +1174,58 @@ static int finish_timing(gpe_handle* h, size_t nev) {
     return 0;
 }
 
+constexpr int kGraphSteps = 64;  // steady-state steps per captured graph (even)
+
+static int ensure_coef(gpe_handle* h, size_t steps) {
+    if (steps <= h->coef_cap) return 0;
+    if (h->rt_graph) {
+        (void)hipGraphExecDestroy(h->rt_graph);
+        h->rt_graph = nullptr;
+    }
+    if (h->coef) GPE_CHECK(hipFree(h->coef));
+    h->coef = nullptr;
+    const size_t cap = std::max<size_t>(steps, 4096);
+    GPE_CHECK(hipMalloc(&h->coef, cap * 2 * sizeof(double)));
+    h->coef_cap = cap;
+    return 0;
+}
+
+// Capture `body` (launches on h->stream) into an executable graph.
+template <class F>
+static int capture(gpe_handle* h, hipGraphExec_t* out, F&& body) {
+    hipGraph_t g = nullptr;
+    GPE_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return 1;
+    }
+    if (e != hipSuccess) return fail(h, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(h, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// One steady-state IT step: Poisson x pass, then the plane pass (finish
+// Poisson, stencil, norm partials, start next Poisson).  Flips cur / pcur.
+template <typename T>
+static int it_step_launch(gpe_handle* h, bool last) {
+    if (launch_line<T>(h, true, false)) return 1;
+    int mode = M_PINV | M_NORM_IN | M_IT | M_NORM_OUT | M_RHO_FWD;
+    if (last) mode = (mode & ~M_RHO_FWD) | M_PHI_STORE;
+    PlaneArgs<T> a = plane_args<T>(h, mode);
+    a.in = (const cplx<T>*)h->store[h->cur];
+    a.out = (cplx<T>*)h->store[1 - h->cur];
+    a.part_in = h->part[h->pcur];
+    a.part_out = h->part[1 - h->pcur];
+    if (launch_plane<T>(h, a)) return 1;
+    h->cur = 1 - h->cur;
+    h->pcur = 1 - h->pcur;
+    return 0;
+}
+
 template <typename T>
 static int do_it_iters(gpe_handle* h, long n) {
     if (n <= 0) return 0;
@@ -1104,11 +1236,35 @@ static int do_it_iters(gpe_handle* h, long n) {
     if (launch_plane<T>(h, a)) return 1;
     size_t ne = 0;
     ev_record(h, ne++);
-    for (long t = 0; t < n; ++t) {
+    long steady = n - 1;
+    if (h->use_graphs && !h->timing && steady >= kGraphSteps) {
+        const int gi = h->cur * 2 + h->pcur;
+        if (h->it_graph[gi] && (h->it_graph_key[gi][0] != h->ggp11 || h->it_graph_key[gi][1] != h->gamma)) {
+            (void)hipGraphExecDestroy(h->it_graph[gi]);
+            h->it_graph[gi] = nullptr;
+        }
+        if (!h->it_graph[gi]) {
+            const int c0 = h->cur, p0 = h->pcur;
+            const int rc = capture(h, &h->it_graph[gi], [&]() -> int {
+                for (int t = 0; t < kGraphSteps; ++t)
+                    if (it_step_launch<T>(h, false)) return 1;
+                return 0;
+            });
+            h->cur = c0;
+            h->pcur = p0;
+            if (rc) return 1;
+            h->it_graph_key[gi][0] = h->ggp11;
+            h->it_graph_key[gi][1] = h->gamma;
+        }
+        while (steady >= kGraphSteps) {
+            GPE_CHECK(hipGraphLaunch(h->it_graph[gi], h->stream));
+            steady -= kGraphSteps;  // an even number of steps: cur / pcur unchanged
+        }
+    }
+    for (long t = 0; t < steady; ++t) {
         if (launch_line<T>(h, true, false)) return 1;
         ev_record(h, ne++);
         int mode = M_PINV | M_NORM_IN | M_IT | M_NORM_OUT | M_RHO_FWD;
-        if (t == n - 1) mode = (mode & ~M_RHO_FWD) | M_PHI_STORE;
         a = plane_args<T>(h, mode);
         a.in = (const cplx<T>*)h->store[h->cur];
         a.out = (cplx<T>*)h->store[1 - h->cur];
@@ -1119,8 +1275,20 @@ static int do_it_iters(gpe_handle* h, long n) {
         h->cur = 1 - h->cur;
         h->pcur = 1 - h->pcur;
     }
+    if (it_step_launch<T>(h, true)) return 1;
     h->norm_pending = true;
     return finish_timing(h, ne);
+}
+
+// Steady-state RT plane pass for step t -> t+1 (coupling from the table).
+template <typename T>
+static PlaneArgs<T> rt_steady_args(gpe_handle* h, int cidx, const int* cbase) {
+    PlaneArgs<T> a = plane_args<T>(h, M_PINV | M_IN_TK | M_PHASE_FULL | M_RHO_FWD | M_KIN);
+    a.cv = 2.0;
+    a.coef = h->coef;
+    a.cbase = cbase;
+    a.cidx = cidx;
+    return a;
 }
 
 template <typename T>
@@ -1129,35 +1297,62 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
     if (materialize<T>(h)) return 1;
     const gpe_params& p = h->p;
     const double dV = p.dx * p.dy * p.dz;
-    auto gq = [&](long t) { return ((g ? g[t] : h->ggp11) - p.cdd / 3) * p.n_atoms / dV; };
-    auto gmq = [&](long t) { return (gm ? gm[t] : h->gamma) * p.w_15 / pow(dV, 1.5); };
+    // per-step scaled couplings (update_edd ramp or constant)
+    if (ensure_coef(h, (size_t)n)) return 1;
+    std::vector<double> cf(2 * (size_t)n);
+    for (long t = 0; t < n; ++t) {
+        cf[2 * t] = ((g ? g[t] : h->ggp11) - p.cdd / 3) * p.n_atoms / dV;
+        cf[2 * t + 1] = (gm ? gm[t] : h->gamma) * p.w_15 / pow(dV, 1.5);
+    }
+    GPE_CHECK(hipMemcpyAsync(h->coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     // prologue: first half phase with the current fi3d, Poisson start, z/y kinetic
     PlaneArgs<T> a = plane_args<T>(h, M_PHI_LOAD | M_PHASE_HALF | M_RHO_FWD | M_KIN);
     a.in = (const cplx<T>*)h->store[h->cur];
-    a.pg = gq(0);
-    a.pgm = gmq(0);
+    a.coef = h->coef;
+    a.cidx = 0;
     if (launch_plane<T>(h, a)) return 1;
     size_t ne = 0;
     ev_record(h, ne++);
-    for (long t = 0; t < n; ++t) {
+    long t = 0;
+    const long steady = n - 1;
+    if (h->use_graphs && !h->timing && steady >= kGraphSteps) {
+        if (!h->rt_graph) {
+            const int rc = capture(h, &h->rt_graph, [&]() -> int {
+                for (int s = 0; s < kGraphSteps; ++s) {
+                    if (launch_line<T>(h, true, true)) return 1;
+                    if (launch_plane<T>(h, rt_steady_args<T>(h, s, h->cbase))) return 1;
+                }
+                hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->cbase, kGraphSteps);
+                GPE_CHECK(hipGetLastError());
+                return 0;
+            });
+            if (rc) return 1;
+        }
+        GPE_CHECK(hipMemsetAsync(h->cbase, 0, sizeof(int), h->stream));
+        while (steady - t >= kGraphSteps) {
+            GPE_CHECK(hipGraphLaunch(h->rt_graph, h->stream));
+            t += kGraphSteps;
+        }
+    }
+    for (; t < steady; ++t) {
         if (launch_line<T>(h, true, true)) return 1;
         ev_record(h, ne++);
-        if (t < n - 1) {
-            a = plane_args<T>(h, M_PINV | M_IN_TK | M_PHASE_FULL | M_RHO_FWD | M_KIN);
-            a.cv = 2.0;
-            a.pg = gq(t) + gq(t + 1);
-            a.pgm = gmq(t) + gmq(t + 1);
-        } else {
-            a = plane_args<T>(h, M_PINV | M_PHI_STORE | M_IN_TK | M_PHASE_HALF | M_OUT);
-            a.out = (cplx<T>*)h->store[h->cur];
-            a.pg = gq(t);
-            a.pgm = gmq(t);
-        }
-        if (launch_plane<T>(h, a)) return 1;
+        if (launch_plane<T>(h, rt_steady_args<T>(h, (int)t, nullptr))) return 1;
         ev_record(h, ne++);
     }
+    // last step: finish with a half phase and write psi, fi3d
+    if (launch_line<T>(h, true, true)) return 1;
+    ev_record(h, ne++);
+    a = plane_args<T>(h, M_PINV | M_PHI_STORE | M_IN_TK | M_PHASE_HALF | M_OUT);
+    a.out = (cplx<T>*)h->store[h->cur];
+    a.coef = h->coef;
+    a.cidx = (int)(n - 1);
+    if (launch_plane<T>(h, a)) return 1;
+    ev_record(h, ne++);
     if (g) h->ggp11 = g[n - 1];
     if (gm) h->gamma = gm[n - 1];
+    // the host-side coupling vector must outlive the async copy
+    GPE_CHECK(hipStreamSynchronize(h->stream));
     return finish_timing(h, ne);
 }
 
@@ -1201,6 +1396,209 @@ static int do_energy(gpe_handle* h, gpe_energies* out) {
     return 0;
 }
 
+
+// ===========================================================================
+// Batch: many instances (grids may differ) stepped together, one launch per
+// pass over all active instances, steady-state chunks replayed as hipGraphs.
+// ===========================================================================
+struct gpe_batch {
+    int prec = GPE_FP32;
+    hipStream_t stream = nullptr;
+    std::vector<gpe_handle*> inst;
+    std::vector<int> active;
+    std::string err;
+    int cur = 0, pcur = 0;  // shared ping-pong parity of every instance
+    // device argument blocks: [0] prologue, [1..4] steady by parity, [5..8] last by parity
+    void* d_pargs = nullptr;
+    void* d_largs = nullptr;
+    void* d_pwork = nullptr;
+    void* d_lwork = nullptr;
+    int npwork = 0, nlwork = 0;
+    size_t plane_lds = 0, line_lds = 0;
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool use_graphs = true;
+};
+
+static int bfail(gpe_batch* b, const std::string& m) {
+    if (b) b->err = m;
+    g_create_err = m;
+    return 1;
+}
+
+#define BCHECK(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return bfail(b, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static void batch_drop_graphs(gpe_batch* b) {
+    for (auto& g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Argument blocks and work lists for the current active set.  Slots:
+// [0..3] prologue, [4..7] steady step, [8..11] last step, indexed by parity
+// (cur*2 + pcur).
+template <typename T>
+static int batch_setup(gpe_batch* b) {
+    batch_drop_graphs(b);
+    const size_t na = b->active.size();
+    std::vector<PlaneArgs<T>> pa(12 * na);
+    std::vector<LineArgs<T>> la(na);
+    std::vector<int2> pw, lw;
+    b->plane_lds = b->line_lds = 0;
+    for (size_t q = 0; q < na; ++q) {
+        gpe_handle* h = b->inst[b->active[q]];
+        for (int par = 0; par < 4; ++par) {
+            const int c = par >> 1, pc = par & 1;
+            PlaneArgs<T> a = plane_args<T>(h, M_RHO_FWD | M_NORM_OUT);
+            a.in = (const cplx<T>*)h->store[c];
+            a.part_out = h->part[pc];
+            pa[(size_t)par * na + q] = a;
+            for (int last = 0; last < 2; ++last) {
+                int mode = M_PINV | M_NORM_IN | M_IT | M_NORM_OUT | M_RHO_FWD;
+                if (last) mode = (mode & ~M_RHO_FWD) | M_PHI_STORE;
+                a = plane_args<T>(h, mode);
+                a.in = (const cplx<T>*)h->store[c];
+                a.out = (cplx<T>*)h->store[1 - c];
+                a.part_in = h->part[pc];
+                a.part_out = h->part[1 - pc];
+                pa[(size_t)(4 + 4 * last + par) * na + q] = a;
+            }
+        }
+        la[q] = line_args<T>(h, true, false);
+        for (int i = 0; i < h->p.nx; ++i) pw.push_back(make_int2((int)q, i));
+        for (int t = 0; t < la[q].ntile_p; ++t) lw.push_back(make_int2((int)q, t));
+        b->plane_lds = std::max(b->plane_lds, h->plane_lds);
+        b->line_lds = std::max(b->line_lds, h->line_lds);
+    }
+    void* old[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork};
+    for (void* o : old)
+        if (o) BCHECK(hipFree(o));
+    b->d_pargs = b->d_largs = nullptr;
+    b->d_pwork = b->d_lwork = nullptr;
+    b->npwork = (int)pw.size();
+    b->nlwork = (int)lw.size();
+    if (na == 0) return 0;
+    BCHECK(hipMalloc(&b->d_pargs, pa.size() * sizeof(PlaneArgs<T>)));
+    BCHECK(hipMalloc(&b->d_largs, la.size() * sizeof(LineArgs<T>)));
+    BCHECK(hipMalloc(&b->d_pwork, pw.size() * sizeof(int2)));
+    BCHECK(hipMalloc(&b->d_lwork, lw.size() * sizeof(int2)));
+    BCHECK(hipMemcpy(b->d_pargs, pa.data(), pa.size() * sizeof(PlaneArgs<T>), hipMemcpyHostToDevice));
+    BCHECK(hipMemcpy(b->d_largs, la.data(), la.size() * sizeof(LineArgs<T>), hipMemcpyHostToDevice));
+    BCHECK(hipMemcpy(b->d_pwork, pw.data(), pw.size() * sizeof(int2), hipMemcpyHostToDevice));
+    BCHECK(hipMemcpy(b->d_lwork, lw.data(), lw.size() * sizeof(int2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+static int batch_plane(gpe_batch* b, int slot) {
+    const PlaneArgs<T>* args = (const PlaneArgs<T>*)b->d_pargs + (size_t)slot * b->active.size();
+    hipLaunchKernelGGL((k_plane_mixed<T, kPlaneThreads>), dim3(b->npwork), dim3(kPlaneThreads), b->plane_lds,
+                       b->stream, args, (const int2*)b->d_pwork);
+    BCHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int batch_line(gpe_batch* b) {
+    hipLaunchKernelGGL((k_xpass_mixed<T, kLineThreads>), dim3(b->nlwork), dim3(kLineThreads), b->line_lds,
+                       b->stream, (const LineArgs<T>*)b->d_largs, (const int2*)b->d_lwork);
+    BCHECK(hipGetLastError());
+    return 0;
+}
+
+// n fused IT steps of every active instance (same semantics as gpe_imag_time_iters).
+template <typename T>
+static int batch_it(gpe_batch* b, long n) {
+    if (n <= 0 || b->active.empty()) return 0;
+    if (batch_plane<T>(b, b->cur * 2 + b->pcur)) return 1;
+    long steady = n - 1;
+    if (b->use_graphs && steady >= kGraphSteps) {
+        const int gi = b->cur * 2 + b->pcur;
+        if (!b->graph[gi]) {
+            hipGraph_t g = nullptr;
+            BCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int c = b->cur, pc = b->pcur, rc = 0;
+            for (int t = 0; t < kGraphSteps && !rc; ++t) {
+                rc = batch_line<T>(b) || batch_plane<T>(b, 4 + c * 2 + pc);
+                c = 1 - c;
+                pc = 1 - pc;
+            }
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return 1;
+            }
+            if (e != hipSuccess) return bfail(b, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph[gi], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return bfail(b, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        while (steady >= kGraphSteps) {
+            BCHECK(hipGraphLaunch(b->graph[gi], b->stream));
+            steady -= kGraphSteps;
+        }
+    }
+    for (long t = 0; t < steady; ++t) {
+        if (batch_line<T>(b) || batch_plane<T>(b, 4 + b->cur * 2 + b->pcur)) return 1;
+        b->cur = 1 - b->cur;
+        b->pcur = 1 - b->pcur;
+    }
+    if (batch_line<T>(b) || batch_plane<T>(b, 8 + b->cur * 2 + b->pcur)) return 1;
+    b->cur = 1 - b->cur;
+    b->pcur = 1 - b->pcur;
+    for (gpe_handle* h : b->inst) {
+        h->cur = b->cur;
+        h->pcur = b->pcur;
+    }
+    for (int q : b->active) b->inst[q]->norm_pending = true;
+    return 0;
+}
+
+template <typename T>
+static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double tol, long* steps_out,
+                          double* e_out) {
+    const int ni = (int)b->inst.size();
+    std::vector<double> last(ni, 0.0);
+    std::vector<char> have(ni, 0);
+    b->active.clear();
+    for (int q = 0; q < ni; ++q) {
+        b->active.push_back(q);
+        if (steps_out) steps_out[q] = 0;
+    }
+    long steps = 0;
+    if (check_every <= 0) check_every = 100;
+    if (batch_setup<T>(b)) return 1;
+    while (!b->active.empty() && steps < max_steps) {
+        const long n = std::min(check_every, max_steps - steps);
+        if (batch_it<T>(b, n)) return 1;
+        steps += n;
+        std::vector<int> keep;
+        for (int q : b->active) {
+            gpe_handle* h = b->inst[q];
+            gpe_energies e;
+            if (do_energy<T>(h, &e)) return bfail(b, h->err);
+            const double epa = e.e_total / h->p.n_atoms;
+            if (e_out) e_out[q] = epa;
+            if (steps_out) steps_out[q] = steps;
+            const bool conv = tol > 0 && have[q] && std::abs(epa - last[q]) < tol * std::abs(epa);
+            last[q] = epa;
+            have[q] = 1;
+            if (!conv) keep.push_back(q);
+        }
+        if (keep.size() != b->active.size()) {
+            b->active = keep;
+            if (batch_setup<T>(b)) return 1;
+        }
+    }
+    BCHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
 #define DISPATCH(h, fn, ...) ((h)->prec == GPE_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
 extern "C" {
@@ -1231,6 +1629,8 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     {
         const char* fg = getenv("GPE_FORCE_GENERIC");
         h->force_generic = fg && fg[0] == '1';
+        const char* ng = getenv("GPE_NO_GRAPH");
+        h->use_graphs = !(ng && ng[0] == '1');
     }
     auto bad = [&](const std::string& m) {
         g_create_err = m;
@@ -1244,7 +1644,7 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     if (hipMalloc(&h->store[0], n * cs) || hipMalloc(&h->store[1], n * cs) || hipMalloc(&h->psii, n * cs) ||
         hipMalloc(&h->V, n * rs) || hipMalloc(&h->phi, n * rs) || hipMalloc(&h->s1, ns1 * cs) ||
         hipMalloc(&h->tk, n * cs) || hipMalloc(&h->part[0], p->nx * 8) || hipMalloc(&h->part[1], p->nx * 8) ||
-        hipMalloc(&h->epart, (size_t)p->nx * 8 * 8))
+        hipMalloc(&h->epart, (size_t)p->nx * 8 * 8) || hipMalloc(&h->cbase, sizeof(int)))
         return bad("device allocation failed");
     if (hipMemset(h->store[0], 0, n * cs) || hipMemset(h->store[1], 0, n * cs) || hipMemset(h->psii, 0, n * cs) ||
         hipMemset(h->V, 0, n * rs) || hipMemset(h->phi, 0, n * rs) || hipMemset(h->part[0], 0, p->nx * 8) ||
@@ -1273,7 +1673,12 @@ void gpe_destroy(gpe_handle* h) {
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->rt_graph) (void)hipGraphExecDestroy(h->rt_graph);
+    for (hipGraphExec_t g : h->it_graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (h->coef) (void)hipFree(h->coef);
+    if (h->cbase) (void)hipFree(h->cbase);
+    if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
 
@@ -1335,6 +1740,72 @@ int gpe_synchronize(gpe_handle* h) {
     GPE_CHECK(hipStreamSynchronize(h->stream));
     return 0;
 }
+int gpe_batch_create(const gpe_params* params, int n, int precision, gpe_batch** out) {
+    if (!params || !out || n <= 0) return fail(nullptr, "bad batch arguments");
+    *out = nullptr;
+    gpe_batch* b = new gpe_batch();
+    b->prec = precision;
+    {
+        const char* ng = getenv("GPE_NO_GRAPH");
+        b->use_graphs = !(ng && ng[0] == '1');
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(nullptr, "stream create failed");
+    }
+    for (int q = 0; q < n; ++q) {
+        gpe_handle* h = nullptr;
+        if (gpe_create(&params[q], precision, &h) != 0) {
+            gpe_batch_destroy(b);
+            return 1;
+        }
+        (void)hipStreamDestroy(h->stream);
+        h->stream = b->stream;
+        h->own_stream = false;
+        b->inst.push_back(h);
+    }
+    *out = b;
+    return 0;
+}
+
+void gpe_batch_destroy(gpe_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
+    for (gpe_handle* h : b->inst) gpe_destroy(h);
+    void* bufs[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+gpe_handle* gpe_batch_instance(gpe_batch* b, int i) {
+    if (!b || i < 0 || i >= (int)b->inst.size()) return nullptr;
+    return b->inst[i];
+}
+
+const char* gpe_batch_last_error(const gpe_batch* b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+int gpe_batch_it_until(gpe_batch* b, long max_steps, long check_every, double tol, long* steps_out,
+                       double* energy_per_atom_out) {
+    if (!b) return fail(nullptr, "null batch");
+    // bring every instance to the same parity: normalised state in buffer 0
+    for (gpe_handle* h : b->inst) {
+        if (DISPATCH(h, materialize, h)) return bfail(b, h->err);
+        if (h->cur != 0) {
+            const size_t bytes = h->N * csize(h);
+            BCHECK(hipMemcpyAsync(h->store[0], h->store[1], bytes, hipMemcpyDeviceToDevice, b->stream));
+            h->cur = 0;
+        }
+        h->pcur = 0;
+    }
+    b->cur = 0;
+    b->pcur = 0;
+    return b->prec == GPE_FP64 ? batch_it_until<double>(b, max_steps, check_every, tol, steps_out, energy_per_atom_out)
+                               : batch_it_until<float>(b, max_steps, check_every, tol, steps_out, energy_per_atom_out);
+}
+
 int gpe_set_timing(gpe_handle* h, int on) {
     if (!h) return 1;
     h->timing = on != 0;

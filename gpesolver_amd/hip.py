@@ -23,6 +23,7 @@ EXPORTS = (
     "gpe_set_coupling", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_synchronize", "gpe_set_timing", "gpe_get_timing",
+    "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
 )
 # Every symbol include/gpesolver_host.h declares.
 HOST_EXPORTS = ("gpes_validate_params_json", "gpes_run")
@@ -90,6 +91,16 @@ def load(path: str = LIB_PATH):
     L.gpe_set_timing.restype = ip
     L.gpe_get_timing.argtypes = [vp, dp, dp, dp]
     L.gpe_get_timing.restype = ip
+    L.gpe_batch_create.argtypes = [C.POINTER(GpeParams), ip, ip, C.POINTER(vp)]
+    L.gpe_batch_create.restype = ip
+    L.gpe_batch_destroy.argtypes = [vp]
+    L.gpe_batch_destroy.restype = None
+    L.gpe_batch_instance.argtypes = [vp, ip]
+    L.gpe_batch_instance.restype = vp
+    L.gpe_batch_last_error.argtypes = [vp]
+    L.gpe_batch_last_error.restype = C.c_char_p
+    L.gpe_batch_it_until.argtypes = [vp, C.c_long, C.c_long, C.c_double, C.POINTER(C.c_long), dp]
+    L.gpe_batch_it_until.restype = ip
     L.gpes_validate_params_json.argtypes = [C.c_char_p, C.c_char_p, ip]
     L.gpes_validate_params_json.restype = ip
     L.gpes_run.argtypes = [C.c_char_p, C.c_double, dp, C.POINTER(C.c_long), C.c_char_p, ip]
@@ -106,11 +117,15 @@ def _dp(a: np.ndarray):
 class Device:
     """One solver instance on the GPU (a gpe_handle)."""
 
-    def __init__(self, params: GpeParams, precision: int = GPE_FP64):
+    def __init__(self, params: GpeParams, precision: int = GPE_FP64, _borrowed=None):
         self.L = load()
         self.params = params
         self.precision = precision
         self.shape = (params.nx, params.ny, params.nz)
+        self._owned = _borrowed is None
+        if _borrowed is not None:
+            self.h = C.c_void_p(_borrowed)
+            return
         h = C.c_void_p()
         rc = self.L.gpe_create(C.byref(params), precision, C.byref(h))
         if rc != 0:
@@ -119,7 +134,8 @@ class Device:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.gpe_destroy(self.h)
+            if self._owned:
+                self.L.gpe_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -230,3 +246,40 @@ def run_directory(workdir: str, it_tol: float = 0.0) -> dict:
     if rc != 0:
         raise GpeError(buf.value.decode())
     return {"energy_per_atom": e.value, "it_steps": n.value}
+
+
+class Batch:
+    """Many instances stepped together (gpe_batch): grids may differ."""
+
+    def __init__(self, params_list, precision: int = GPE_FP32):
+        self.L = load()
+        arr = (GpeParams * len(params_list))(*params_list)
+        b = C.c_void_p()
+        rc = self.L.gpe_batch_create(arr, len(params_list), precision, C.byref(b))
+        if rc != 0:
+            raise GpeError(self.L.gpe_last_error(None).decode())
+        self.b = b
+        self.instances = [Device(p, precision, _borrowed=self.L.gpe_batch_instance(b, i))
+                          for i, p in enumerate(params_list)]
+
+    def close(self):
+        if getattr(self, "b", None):
+            for d in self.instances:
+                d.h = None
+            self.L.gpe_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def it_until(self, max_steps: int, check_every: int = 100, tol: float = 0.0):
+        n = len(self.instances)
+        steps = (C.c_long * n)()
+        e = np.zeros(n)
+        rc = self.L.gpe_batch_it_until(self.b, max_steps, check_every, tol, steps, _dp(e))
+        if rc != 0:
+            raise GpeError(self.L.gpe_batch_last_error(self.b).decode())
+        return np.array(steps[:], dtype=np.int64), e

@@ -92,6 +92,24 @@ int gpe_calc_energy(gpe_handle *h, gpe_energies *out); /* cpu_solver.cpp:137-194
 int gpe_imag_time_iters(gpe_handle *h, long n);
 int gpe_real_time_iters(gpe_handle *h, long n, const double *ggp11, const double *gamma);
 
+/* Batch (no reference counterpart: the reference is single-instance with
+ * singleton parameters, src/parameters/parameters.hpp:14-20).  n instances,
+ * each with its own constants and grid (grids may differ), share one stream;
+ * every pass of a step is one launch over all active instances and the
+ * steady-state steps are replayed as hipGraphs.  gpe_batch_instance gives the
+ * per-instance handle for load/export/energy calls. */
+typedef struct gpe_batch gpe_batch;
+int gpe_batch_create(const gpe_params *params, int n, int precision, gpe_batch **out);
+void gpe_batch_destroy(gpe_batch *b);
+gpe_handle *gpe_batch_instance(gpe_batch *b, int i);
+const char *gpe_batch_last_error(const gpe_batch *b);
+/* Imaginary time on every instance until |E/N(k) - E/N(k - check_every)| <
+ * tol |E/N(k)| (energies checked every check_every steps) or max_steps; a
+ * converged instance leaves the launch set.  steps_out / energy_per_atom_out
+ * (length n, may be NULL) receive each instance's step count and final E/N. */
+int gpe_batch_it_until(gpe_batch *b, long max_steps, long check_every, double tol, long *steps_out,
+                       double *energy_per_atom_out);
+
 /* Wait for all work of h. */
 int gpe_synchronize(gpe_handle *h);
 

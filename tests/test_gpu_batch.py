@@ -1,0 +1,73 @@
+"""Batched imaginary time: several instances (different constants and grids)
+stepped by one launch per pass, against the single-instance path and the
+oracle, and the per-instance convergence test."""
+import numpy as np
+import pytest
+
+from gpesolver_amd import hip
+from gpesolver_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((32, 24, 12), 1.5, 2e4), ((16, 16, 8), 1.3, 8e3), ((32, 24, 12), 1.45, 3e4), ((24, 12, 6), 1.4, 1e4)]
+
+
+def setup_instance(shape, edd, n_atoms):
+    nx, ny, nz = shape
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, 120.0, 120.0, 600.0, n_atoms=n_atoms, edd=edd)
+    V = P.potential("HARMONIC", p)
+    psi = P.add_phase_noise(P.wavefunction("GAUSS", p), 1e-3, 11)
+    return p, V, psi
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("prec", [hip.GPE_FP64, hip.GPE_FP32], ids=["fp64", "fp32"])
+def test_batch_matches_single_and_oracle(oracle, prec):
+    setups = [setup_instance(*c) for c in CASES]
+    b = hip.Batch([s[0].to_gpe() for s in setups], prec)
+    for d, (p, V, psi) in zip(b.instances, setups):
+        d.load_pote(V)
+        d.load_buffer(psi)
+    steps, e = b.it_until(100, check_every=50, tol=0.0)
+    assert list(steps) == [100] * len(CASES)
+    for q, (p, V, psi) in enumerate(setups):
+        single = hip.Device(p.to_gpe(), prec)
+        single.load_pote(V)
+        single.load_buffer(psi)
+        single.it_steps(50)
+        single.energy()
+        single.it_steps(50)
+        es = single.energy()
+        got = b.instances[q].export_psi()
+        assert rel(got, single.export_psi()) < (1e-14 if prec == hip.GPE_FP64 else 1e-6)
+        assert abs(e[q] - es["e_total"] / p.n_atoms) <= 1e-12 * abs(e[q]) + (0 if prec == hip.GPE_FP64 else 1e-6 * abs(e[q]))
+        op = oracle.make_params(p.nx, p.ny, p.nz, 120.0, 120.0, 600.0, n_atoms=p.n_atoms, edd=p.edd)
+        o = oracle.Oracle(op)
+        o.load_pote(V)
+        o.load_buffer(psi)
+        o.it_steps(100)
+        assert rel(got, o.psi) < (1e-10 if prec == hip.GPE_FP64 else 1e-5)
+        single.close()
+    b.close()
+
+
+def test_batch_convergence_drops_instances():
+    setups = [setup_instance(*c) for c in CASES]
+    b = hip.Batch([s[0].to_gpe() for s in setups], hip.GPE_FP32)
+    for d, (p, V, psi) in zip(b.instances, setups):
+        d.load_pote(V)
+        d.load_buffer(psi)
+    steps, e = b.it_until(3000, check_every=100, tol=1e-4)
+    assert np.all(steps % 100 == 0) and np.all(steps <= 3000) and np.all(steps >= 200)
+    assert len(set(steps.tolist())) > 1  # instances converge at different steps
+    # each converged instance satisfies the criterion at its last check
+    for q, d in enumerate(b.instances):
+        if steps[q] < 3000:
+            e1 = d.energy()["e_total"]
+            d.it_steps(100)
+            e2 = d.energy()["e_total"]
+            assert abs(e2 - e1) < 1e-4 * abs(e2) * 1.5
+    b.close()
