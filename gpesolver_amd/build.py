@@ -9,9 +9,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libgpehip.so")
 EXE = os.path.join(HERE, "gpesolver")
-SOURCES = [os.path.join(HERE, "csrc", "gpehip.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "gpehip.hip"), os.path.join(HERE, "csrc", "gpe_fno.hip")]
 HOST_SOURCES = [os.path.join(HERE, "host", "gpe_host.cpp"), os.path.join(HERE, "host", "gpe_host_abi.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "gpe_fft.hpp"), os.path.join(HERE, "csrc", "gpe_fft_ct.hpp"),
+           os.path.join(HERE, "csrc", "gpe_gemm.hpp"),
            os.path.join(HERE, "host", "gpe_host.hpp"), os.path.join(ROOT, "include", "gpehip.h"),
            os.path.join(ROOT, "include", "gpesolver_host.h")]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]

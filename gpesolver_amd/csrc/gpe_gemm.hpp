@@ -1,0 +1,183 @@
+// gpe_gemm.hpp -- strided batched (complex) GEMM on gfx950 BF16 MFMA.
+//
+//   C[z] = alpha * op(A[z]) * op(B[z]) (+ bias[n]) + beta * C[z]
+//
+// A (m,k) lives at A[z*saz + m*sam + k*sak], B (k,n) at B[z*sbk...] (any
+// strides, so transposes are free), C (m,n) at C[z*scz + m*scm + n*scn].
+// Operands are FP32 in memory and rounded to BF16 (round to nearest even) as
+// they are staged into LDS; products accumulate in FP32 in the MFMA unit.
+// Complex operands are (re, im) pointer pairs with the same strides; an
+// imaginary pointer of nullptr means a real operand, and Ci == nullptr writes
+// only Re(C).  conjA / conjB conjugate the operand.
+//
+// Tile: 64x64 per block of 256 threads (4 waves, 32x32 each = 2x2 MFMA
+// 16x16x32 tiles), K staged 32 at a time.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gpe {
+namespace gemm {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct Args {
+    int M, N, K, batch;
+    const float* Ar;
+    const float* Ai;
+    long sam, sak, saz;
+    const float* Br;
+    const float* Bi;
+    long sbk, sbn, sbz;
+    float* Cr;
+    float* Ci;
+    long scm, scn, scz;
+    const float* bias;  // added to Re(C) per column n (may be null)
+    float alpha, beta;
+    int conjA, conjB;
+};
+
+constexpr int TM = 64, TN = 64, TK = 32, NT = 256;
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned int u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));  // inf/nan
+    const unsigned int rnd = 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)((u + rnd) >> 16);
+}
+
+// LDS tiles hold bf16 bit patterns: A as [m][k], B transposed as [n][k], so the
+// 8 consecutive k values an MFMA lane needs are one 16-byte read.
+struct Smem {
+    unsigned short ar[TM][TK + 8];
+    unsigned short ai[TM][TK + 8];
+    unsigned short br[TN][TK + 8];
+    unsigned short bi[TN][TK + 8];
+};
+
+__device__ __forceinline__ bf16x8 ld8(const unsigned short* p) {
+    bf16x8 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+
+__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
+    unsigned short t[8];
+    __builtin_memcpy(t, &v, 16);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) t[i] ^= 0x8000;
+    __builtin_memcpy(&v, t, 16);
+    return v;
+}
+
+__global__ void __launch_bounds__(NT) k_gemm(Args a) {
+    __shared__ Smem sm;
+    const int z = blockIdx.z;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const bool cA = a.Ai != nullptr, cB = a.Bi != nullptr;
+    const bool cplx_out = a.Ci != nullptr;
+    const float sgnA = a.conjA ? -1.f : 1.f, sgnB = a.conjB ? -1.f : 1.f;
+    const float* Ar = a.Ar + (long)z * a.saz;
+    const float* Ai = cA ? a.Ai + (long)z * a.saz : nullptr;
+    const float* Br = a.Br + (long)z * a.sbz;
+    const float* Bi = cB ? a.Bi + (long)z * a.sbz : nullptr;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acci[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+
+    for (int k0 = 0; k0 < a.K; k0 += TK) {
+        // stage A (64 x 32) and B (32 x 64): 8 elements per thread each
+#pragma unroll
+        for (int q = 0; q < (TM * TK) / NT; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int mm = e / TK, kk = e % TK;
+            const int m = m0 + mm, k = k0 + kk;
+            float vr = 0.f, vi = 0.f;
+            if (m < a.M && k < a.K) {
+                const long off = (long)m * a.sam + (long)k * a.sak;
+                vr = Ar[off];
+                if (cA) vi = sgnA * Ai[off];
+            }
+            sm.ar[mm][kk] = f2bf(vr);
+            sm.ai[mm][kk] = f2bf(vi);
+        }
+#pragma unroll
+        for (int q = 0; q < (TN * TK) / NT; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int nn = e / TK, kk = e % TK;
+            const int n = n0 + nn, k = k0 + kk;
+            float vr = 0.f, vi = 0.f;
+            if (n < a.N && k < a.K) {
+                const long off = (long)k * a.sbk + (long)n * a.sbn;
+                vr = Br[off];
+                if (cB) vi = sgnB * Bi[off];
+            }
+            sm.br[nn][kk] = f2bf(vr);
+            sm.bi[nn][kk] = f2bf(vi);
+        }
+        __syncthreads();
+        // MFMA 16x16x32: lane holds A[m = l%16][k = 8*(l/16) + 0..7] and
+        // B[k = 8*(l/16) + 0..7][n = l%16].
+        const int r16 = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const bf16x8 fa_r = ld8(&sm.ar[wm + 16 * i + r16][kq]);
+            bf16x8 fa_i, fa_in;
+            if (cA) {
+                fa_i = ld8(&sm.ai[wm + 16 * i + r16][kq]);
+                fa_in = neg8(fa_i);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const bf16x8 fb_r = ld8(&sm.br[wn + 16 * j + r16][kq]);
+                accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_r, fb_r, accr[i][j], 0, 0, 0);
+                if (cB) {
+                    const bf16x8 fb_i = ld8(&sm.bi[wn + 16 * j + r16][kq]);
+                    if (cA) accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_in, fb_i, accr[i][j], 0, 0, 0);
+                    if (cplx_out)
+                        acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_r, fb_i, acci[i][j], 0, 0, 0);
+                }
+                if (cA && cplx_out)
+                    acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa_i, fb_r, acci[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D[m = 4*(l/16) + r][n = l%16]
+    float* Cr = a.Cr + (long)z * a.scz;
+    float* Ci = cplx_out ? a.Ci + (long)z * a.scz : nullptr;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + 16 * j + (lane & 15);
+                if (m < a.M && n < a.N) {
+                    const long off = (long)m * a.scm + (long)n * a.scn;
+                    float vr = a.alpha * accr[i][j][r];
+                    if (a.bias) vr += a.bias[n];
+                    if (a.beta != 0.f) vr += a.beta * Cr[off];
+                    Cr[off] = vr;
+                    if (cplx_out) {
+                        float vi = a.alpha * acci[i][j][r];
+                        if (a.beta != 0.f) vi += a.beta * Ci[off];
+                        Ci[off] = vi;
+                    }
+                }
+            }
+}
+
+}  // namespace gemm
+}  // namespace gpe

@@ -265,6 +265,10 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
     cplx<T>* P = reinterpret_cast<cplx<T>*>(lds_raw);
     const int nx = a.nx, ny = Pol::ny(a), nz = Pol::nz(a);
     const int nzh = nz + 1, nz2 = 2 * nz, ny2 = 2 * ny;
+    // LDS row strides padded to an odd number of complex elements so that
+    // consecutive rows start on different banks
+    const int zs = nz2 + 1;  // z-line rows of the R2C/C2R stage
+    const int qs = nz + 1;   // psi plane rows
     const int npl = ny * nz;
     const int mode = a.mode;
     const size_t plane_off = (size_t)i * npl;
@@ -297,19 +301,19 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
             const int e = threadIdx.x + q * NT;
             if (e < ny * nzh) {
                 const int j = e / nzh, kz = e - j * nzh;
-                P[j * nz2 + kz] = tmp[q];
-                if (kz >= 1 && kz < nz) P[j * nz2 + (nz2 - kz)] = gpe::mk(tmp[q].x, -tmp[q].y);
+                P[j * zs + kz] = tmp[q];
+                if (kz >= 1 && kz < nz) P[j * zs + (nz2 - kz)] = gpe::mk(tmp[q].x, -tmp[q].y);
             }
         }
         __syncthreads();
-        Pol::template fft_z2<T, NT, +1>(P, ny, 1, nz2, a);
+        Pol::template fft_z2<T, NT, +1>(P, ny, 1, zs, a);
         const T sc = (T)(a.pinv_scale * s_in * s_in);
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int p = threadIdx.x + q * NT;
             if (p < npl) {
                 const int j = p / nz, k = p - j * nz;
-                phi_r[q] = P[j * nz2 + k].x * sc;
+                phi_r[q] = P[j * zs + k].x * sc;
                 if (mode & M_PHI_STORE) a.phi[plane_off + p] = phi_r[q];
             }
         }
@@ -330,7 +334,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         const cplx<T>* in = a.in;
         for (int p = threadIdx.x; p < npl; p += NT) {
             cplx<T> v = in[plane_off + p];
-            Q[p] = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+            Q[(p / nz) * qs + p % nz] = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
         }
         __syncthreads();
         const T sv = (T)s_in;
@@ -344,9 +348,10 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
                 const int j = p / nz, k = p - j * nz;
                 cplx<T> r;
                 if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
-                    const cplx<T> c = Q[p];
+                    const int qi = j * qs + k;
+                    const cplx<T> c = Q[qi];
                     const cplx<T> xm = in[plane_off - npl + p], xp = in[plane_off + npl + p];
-                    const cplx<T> ym = Q[p - nz], yp = Q[p + nz], zm = Q[p - 1], zp = Q[p + 1];
+                    const cplx<T> ym = Q[qi - qs], yp = Q[qi + qs], zm = Q[qi - 1], zp = Q[qi + 1];
                     const T v = a.V[plane_off + p] + cdd * phi_r[q];
                     cplx<T> c1;
                     c1.x = cx * (xm.x * sv + xp.x * sv - T(2) * c.x) + cy * (ym.x + yp.x - T(2) * c.x) +
@@ -375,13 +380,13 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         if (mode & M_IN_TK) {
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
-                Q[p] = a.tk[((size_t)j * nx + i) * nz + k];
+                Q[j * qs + k] = a.tk[((size_t)j * nx + i) * nz + k];
             }
         } else {
             for (int p = threadIdx.x; p < npl; p += NT) {
                 cplx<T> v = a.in[plane_off + p];
                 if (mode & M_NORM_IN) v = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
-                Q[p] = v;
+                Q[(p / nz) * qs + p % nz] = v;
             }
         }
         __syncthreads();
@@ -400,18 +405,24 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         for (int q = 0; q < PPT; ++q) {
             const int p = threadIdx.x + q * NT;
             if (p < npl) {
-                cplx<T> c = Q[p];
+                const int qi = (p / nz) * qs + p % nz;
+                cplx<T> c = Q[qi];
                 T nrm = c.x * c.x + c.y * c.y;
                 if (mode & (M_PHASE_HALF | M_PHASE_FULL)) {
                     const int j = p / nz, k = p - j * nz;
                     if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
-                        const T ab = sqrt(nrm);
-                        const T veff = (T)a.cv * (a.V[plane_off + p] + (T)a.cdd * phi_r[q]) + (T)pg * nrm +
-                                       (T)pgm * (ab * ab * ab);
+                        // phase argument in FP64 and reduced to [-pi, pi] before
+                        // the sincos of the bulk precision, so a large phase
+                        // (dense states, long steps) keeps its absolute accuracy
+                        const double dn = (double)nrm;
+                        const double veff = a.cv * ((double)a.V[plane_off + p] + a.cdd * (double)phi_r[q]) +
+                                            pg * dn + pgm * dn * sqrt(dn);
+                        double th = -a.hdt * veff;
+                        th -= 6.283185307179586 * rint(th * 0.15915494309189535);
                         T sn, cs;
-                        sincos_t<T>(-(T)a.hdt * veff, &sn, &cs);
+                        sincos_t<T>((T)th, &sn, &cs);
                         c = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
-                        Q[p] = c;
+                        Q[qi] = c;
                         nrm = c.x * c.x + c.y * c.y;
                     }
                 }
@@ -421,22 +432,28 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         }
         __syncthreads();
         if (mode & M_OUT) {
-            for (int p = threadIdx.x; p < npl; p += NT) a.out[plane_off + p] = Q[p];
+            for (int p = threadIdx.x; p < npl; p += NT) a.out[plane_off + p] = Q[(p / nz) * qs + p % nz];
         }
         if (mode & M_KIN) {
             // z propagation: rows j, length nz
-            Pol::template fft_z<T, NT, -1>(Q, ny, 1, nz, a);
-            for (int p = threadIdx.x; p < npl; p += NT) Q[p] = gpe::cmul(Q[p], a.ez[p % nz]);
+            Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a);
+            for (int p = threadIdx.x; p < npl; p += NT) {
+                const int qi = (p / nz) * qs + p % nz;
+                Q[qi] = gpe::cmul(Q[qi], a.ez[p % nz]);
+            }
             __syncthreads();
-            Pol::template fft_z<T, NT, +1>(Q, ny, 1, nz, a);
+            Pol::template fft_z<T, NT, +1>(Q, ny, 1, qs, a);
             // y propagation: columns k, length ny
-            Pol::template fft_y<T, NT, -1>(Q, nz, nz, 1, a);
-            for (int p = threadIdx.x; p < npl; p += NT) Q[p] = gpe::cmul(Q[p], a.ey[p / nz]);
+            Pol::template fft_y<T, NT, -1>(Q, nz, qs, 1, a);
+            for (int p = threadIdx.x; p < npl; p += NT) {
+                const int qi = (p / nz) * qs + p % nz;
+                Q[qi] = gpe::cmul(Q[qi], a.ey[p / nz]);
+            }
             __syncthreads();
-            Pol::template fft_y<T, NT, +1>(Q, nz, nz, 1, a);
+            Pol::template fft_y<T, NT, +1>(Q, nz, qs, 1, a);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
-                a.tk[((size_t)j * nx + i) * nz + k] = Q[p];
+                a.tk[((size_t)j * nx + i) * nz + k] = Q[j * qs + k];
             }
         }
         __syncthreads();
@@ -455,19 +472,19 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
             const int p = threadIdx.x + q * NT;
             if (p < npl) {
                 const int j = p / nz, k = p - j * nz;
-                P[j * nz2 + k] = gpe::mk(rho_r[q], T(0));
-                P[j * nz2 + nz + k] = gpe::mk(T(0), T(0));
+                P[j * zs + k] = gpe::mk(rho_r[q], T(0));
+                P[j * zs + nz + k] = gpe::mk(T(0), T(0));
             }
         }
         __syncthreads();
-        Pol::template fft_z2<T, NT, -1>(P, ny, 1, nz2, a);
+        Pol::template fft_z2<T, NT, -1>(P, ny, 1, zs, a);
         cplx<T> tmp[RQ];
 #pragma unroll
         for (int q = 0; q < RQ; ++q) {
             const int e = threadIdx.x + q * NT;
             if (e < ny * nzh) {
                 const int j = e / nzh, kz = e - j * nzh;
-                tmp[q] = P[j * nz2 + kz];
+                tmp[q] = P[j * zs + kz];
             }
         }
         __syncthreads();
@@ -497,26 +514,28 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
         const int kz0 = (b - ky * a.tiles_kz) * a.cp;
         const int C = min(a.cp, nzh - kz0);
         const int nx2 = 2 * nx;
+        const int CS = C + 1;  // padded LDS stride between x points
         cplx<T>* base = a.s1 + (size_t)ky * nx * nzh + kz0;
         for (int e = threadIdx.x; e < nx2 * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            X[e] = (ii < nx) ? base[(size_t)ii * nzh + c] : gpe::mk(T(0), T(0));
+            X[ii * CS + c] = (ii < nx) ? base[(size_t)ii * nzh + c] : gpe::mk(T(0), T(0));
         }
         __syncthreads();
-        Pol::template fft_x2<T, NT, -1>(X, C, C, 1, a);
-        const double ky2 = a.ky2[ky];
+        Pol::template fft_x2<T, NT, -1>(X, C, CS, 1, a);
+        // Vdd(k) = kz^2 / k^2 from the FP64 per-axis tables, formed in T
+        const T ky2 = (T)a.ky2[ky];
         for (int e = threadIdx.x; e < nx2 * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            const double kz2 = a.kz2[kz0 + c];
-            const double k2 = a.kx2[ii] + ky2 + kz2;
-            const T v = (k2 > 1e-30) ? (T)(kz2 / k2) : T(0);
-            X[e] = gpe::cscale(X[e], v);
+            const T kz2 = (T)a.kz2[kz0 + c];
+            const T k2 = (T)a.kx2[ii] + ky2 + kz2;
+            const T v = (k2 > T(1e-30)) ? kz2 / k2 : T(0);
+            X[ii * CS + c] = gpe::cscale(X[ii * CS + c], v);
         }
         __syncthreads();
-        Pol::template fft_x2<T, NT, +1>(X, C, C, 1, a);
+        Pol::template fft_x2<T, NT, +1>(X, C, CS, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            base[(size_t)ii * nzh + c] = X[e];
+            base[(size_t)ii * nzh + c] = X[ii * CS + c];
         }
     } else {
         b -= a.ntile_p;
@@ -524,19 +543,23 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
         const int j = b / a.tiles_k;
         const int k0 = (b - j * a.tiles_k) * a.ck;
         const int C = min(a.ck, nz - k0);
+        const int CS = C + 1;
         cplx<T>* base = a.tk + (size_t)j * nx * nz + k0;
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            X[e] = base[(size_t)ii * nz + c];
+            X[ii * CS + c] = base[(size_t)ii * nz + c];
         }
         __syncthreads();
-        Pol::template fft_x<T, NT, -1>(X, C, C, 1, a);
-        for (int e = threadIdx.x; e < nx * C; e += NT) X[e] = gpe::cmul(X[e], a.ex[e / C]);
-        __syncthreads();
-        Pol::template fft_x<T, NT, +1>(X, C, C, 1, a);
+        Pol::template fft_x<T, NT, -1>(X, C, CS, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            base[(size_t)ii * nz + c] = X[e];
+            X[ii * CS + c] = gpe::cmul(X[ii * CS + c], a.ex[ii]);
+        }
+        __syncthreads();
+        Pol::template fft_x<T, NT, +1>(X, C, CS, 1, a);
+        for (int e = threadIdx.x; e < nx * C; e += NT) {
+            const int ii = e / C, c = e - ii * C;
+            base[(size_t)ii * nz + c] = X[ii * CS + c];
         }
     }
 }
@@ -643,11 +666,12 @@ __global__ void k_potential_half(cplx<T>* psi, const T* V, const T* phi, int nx,
         const int i = e / ((size_t)ny * nz);
         if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > nz - 2) continue;
         cplx<T> c = psi[e];
-        const T nrm = c.x * c.x + c.y * c.y;
-        const T ab = sqrt(nrm);
-        const T veff = V[e] + (T)cdd * phi[e] + (T)pg * nrm + (T)pgm * (ab * ab * ab);
+        const double nrm = (double)(c.x * c.x + c.y * c.y);
+        const double veff = (double)V[e] + cdd * (double)phi[e] + pg * nrm + pgm * nrm * sqrt(nrm);
+        double th = -hdt * veff;
+        th -= 6.283185307179586 * rint(th * 0.15915494309189535);
         T sn, cs;
-        sincos_t<T>(-(T)hdt * veff, &sn, &cs);
+        sincos_t<T>((T)th, &sn, &cs);
         psi[e] = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
     }
 }
@@ -1658,8 +1682,8 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     h->cp = std::min(h->cp, 12);
     h->ck = std::min(p->nz, std::max(1, kLineMaxE / p->nx));
     h->ck = std::min(h->ck, 16);
-    h->plane_lds = (size_t)std::max(2L * p->ny * (p->nz + 1), (long)p->ny * p->nz) * cs;
-    h->line_lds = (size_t)std::max(2L * p->nx * h->cp, (long)p->nx * h->ck) * cs;
+    h->plane_lds = (size_t)std::max(2L * p->ny * (p->nz + 1), (long)p->ny * (2 * p->nz + 1)) * cs;
+    h->line_lds = (size_t)std::max(2L * p->nx * (h->cp + 1), (long)p->nx * (h->ck + 1)) * cs;
     if (hipDeviceSynchronize() != hipSuccess) return bad("device synchronize failed after setup");
     *out = h;
     return 0;

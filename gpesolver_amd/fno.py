@@ -1,0 +1,152 @@
+"""Fourier-neural-operator warm start for imaginary time (ctypes binding of
+the gpe_fno_* entry points of include/gpehip.h) plus the feature / target
+conventions shared by training and inference.
+
+Input channels per grid point (cin = 6):
+  0  V / V0                      potential in units of V0 = 1e-12 hartree
+  1  (eps_dd - 1.4) / 0.2
+  2  (log10 N - 4.35) / 0.5
+  3  (omega_z[Hz] - 120) / 20
+  4  potential type (0 HARMONIC, 1 MEXICAN)
+  5  dd / 1500 nm
+Target: the ground-state amplitude |psi| / max|psi| (real, >= 0).  The warm
+start is psi0 = max(prediction, 0), which gpe_load_buffer normalises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import hip
+from . import params as P
+
+
+class FnoConfig(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("nx", "ny", "nz", "cin", "width", "layers", "hidden", "modes", "max_batch")]
+
+
+CIN = 6
+V0 = 1e-12
+
+
+def _bind():
+    L = hip.load()
+    vp, ip, fp, dp = C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_double)
+    L.gpe_fno_create.argtypes = [C.POINTER(FnoConfig), C.c_ulonglong, C.POINTER(vp)]
+    L.gpe_fno_create.restype = ip
+    L.gpe_fno_destroy.argtypes = [vp]
+    L.gpe_fno_destroy.restype = None
+    L.gpe_fno_last_error.argtypes = [vp]
+    L.gpe_fno_last_error.restype = C.c_char_p
+    L.gpe_fno_num_params.argtypes = [vp]
+    L.gpe_fno_num_params.restype = C.c_long
+    for n in ("gpe_fno_get_params", "gpe_fno_set_params", "gpe_fno_get_grads"):
+        getattr(L, n).argtypes = [vp, fp]
+        getattr(L, n).restype = ip
+    L.gpe_fno_forward.argtypes = [vp, fp, ip, fp]
+    L.gpe_fno_forward.restype = ip
+    L.gpe_fno_loss_grad.argtypes = [vp, fp, fp, ip, dp]
+    L.gpe_fno_loss_grad.restype = ip
+    L.gpe_fno_adam_step.argtypes = [vp, C.c_float]
+    L.gpe_fno_adam_step.restype = ip
+    return L
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Fno:
+    def __init__(self, nx=64, ny=64, nz=16, width=32, layers=4, hidden=64, modes=16, max_batch=8, seed=1, cin=CIN):
+        self.L = _bind()
+        self.cfg = FnoConfig(nx, ny, nz, cin, width, layers, hidden, modes, max_batch)
+        f = C.c_void_p()
+        rc = self.L.gpe_fno_create(C.byref(self.cfg), seed, C.byref(f))
+        if rc != 0:
+            msg = self.L.gpe_fno_last_error(f).decode() if f else "gpe_fno_create failed"
+            if f:
+                self.L.gpe_fno_destroy(f)
+            raise hip.GpeError(msg)
+        self.f = f
+        self.P = nx * ny * nz
+
+    def close(self):
+        if getattr(self, "f", None):
+            self.L.gpe_fno_destroy(self.f)
+            self.f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise hip.GpeError(self.L.gpe_fno_last_error(self.f).decode())
+
+    @property
+    def num_params(self) -> int:
+        return self.L.gpe_fno_num_params(self.f)
+
+    def get_params(self) -> np.ndarray:
+        w = np.empty(self.num_params, dtype=np.float32)
+        self._check(self.L.gpe_fno_get_params(self.f, _fp(w)))
+        return w
+
+    def set_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.num_params
+        self._check(self.L.gpe_fno_set_params(self.f, _fp(w)))
+
+    def get_grads(self) -> np.ndarray:
+        g = np.empty(self.num_params, dtype=np.float32)
+        self._check(self.L.gpe_fno_get_grads(self.f, _fp(g)))
+        return g
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        B = x.shape[0]
+        y = np.empty((B, self.P), dtype=np.float32)
+        self._check(self.L.gpe_fno_forward(self.f, _fp(x), B, _fp(y)))
+        return y
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.L.gpe_fno_loss_grad(self.f, _fp(x), _fp(t), x.shape[0], C.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr: float):
+        self._check(self.L.gpe_fno_adam_step(self.f, lr))
+
+
+def features(p: P.PhysicalParameters, V: np.ndarray, pot: str) -> np.ndarray:
+    """Per-point input channels [P, CIN] for one instance (see module doc)."""
+    n = V.size
+    x = np.empty((n, CIN), dtype=np.float32)
+    x[:, 0] = (V.reshape(-1) / V0).astype(np.float32)
+    x[:, 1] = (p.edd - 1.4) / 0.2
+    x[:, 2] = (math.log10(p.n_atoms) - 4.35) / 0.5
+    x[:, 3] = (P.freq_au_to_Hz(p.omega_z) - 120.0) / 20.0
+    x[:, 4] = 1.0 if pot.startswith("MEXICAN") else 0.0
+    x[:, 5] = P.len_au_to_nm(p.dd) / 1500.0
+    return x
+
+
+def target(psi: np.ndarray) -> np.ndarray:
+    a = np.abs(psi).reshape(-1)
+    return (a / a.max()).astype(np.float32)
+
+
+def warm_state(pred: np.ndarray, shape) -> np.ndarray:
+    """Initial state from a prediction: interior amplitude, boundary zero
+    (like the reference's generated initial states)."""
+    a = np.maximum(pred.reshape(shape).astype(np.float64), 0.0)
+    psi = np.zeros(shape, dtype=np.complex128)
+    psi[1:-1, 1:-1, 1:-1] = a[1:-1, 1:-1, 1:-1]
+    return psi

@@ -24,6 +24,9 @@ EXPORTS = (
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_synchronize", "gpe_set_timing", "gpe_get_timing",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
+    "gpe_fno_create", "gpe_fno_destroy", "gpe_fno_last_error", "gpe_fno_num_params", "gpe_fno_get_params",
+    "gpe_fno_set_params", "gpe_fno_get_grads", "gpe_fno_forward", "gpe_fno_loss_grad", "gpe_fno_adam_step",
+    "gpe_gemm_test",
 )
 # Every symbol include/gpesolver_host.h declares.
 HOST_EXPORTS = ("gpes_validate_params_json", "gpes_run")

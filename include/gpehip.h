@@ -110,6 +110,36 @@ const char *gpe_batch_last_error(const gpe_batch *b);
 int gpe_batch_it_until(gpe_batch *b, long max_steps, long check_every, double tol, long *steps_out,
                        double *energy_per_atom_out);
 
+/* Learned warm start (no reference counterpart): a Fourier neural operator
+ * mapping per-point parameter fields [B][nx*ny*nz][cin] to a field
+ * [B][nx*ny*nz] (the ground-state amplitude, used as the IT initial state via
+ * gpe_load_buffer).  Every layer runs as BF16-MFMA GEMMs with FP32
+ * accumulation; weights are FP32 masters rounded to BF16 at use. */
+typedef struct {
+    int nx, ny, nz;
+    int cin;       /* input channels */
+    int width;     /* channel width of the spectral layers */
+    int layers;    /* number of spectral layers */
+    int hidden;    /* projection hidden width */
+    int modes;     /* retained modes per axis (capped at nx, ny and nz/2+1) */
+    int max_batch; /* samples per call */
+} gpe_fno_config;
+typedef struct gpe_fno gpe_fno;
+int gpe_fno_create(const gpe_fno_config *c, unsigned long long seed, gpe_fno **out);
+void gpe_fno_destroy(gpe_fno *f);
+const char *gpe_fno_last_error(const gpe_fno *f);
+long gpe_fno_num_params(const gpe_fno *f);
+int gpe_fno_get_params(gpe_fno *f, float *host);
+int gpe_fno_set_params(gpe_fno *f, const float *host);
+int gpe_fno_get_grads(gpe_fno *f, float *host);
+int gpe_fno_forward(gpe_fno *f, const float *x, int batch, float *y_out);
+int gpe_fno_loss_grad(gpe_fno *f, const float *x, const float *target, int batch, double *loss);
+int gpe_fno_adam_step(gpe_fno *f, float lr);
+/* C = op(A) op(B) on the MFMA GEMM (row-major host matrices; imaginary parts
+ * may be NULL; Ci NULL keeps only the real part).  Test entry point. */
+int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float *Bi, float *Cr, float *Ci, int M,
+                  int N, int K, int conjA, int conjB);
+
 /* Wait for all work of h. */
 int gpe_synchronize(gpe_handle *h);
 

@@ -1,0 +1,78 @@
+"""The BF16-MFMA GEMM (gpesolver_amd/csrc/gpe_gemm.hpp) against numpy on
+BF16-rounded operands (so only FP32 accumulation order differs), for real and
+complex operands, conjugation, odd sizes and K not a multiple of the tile."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpesolver_amd import hip
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """Round-to-nearest-even to bfloat16, returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def run(Ar, Ai, Br, Bi, cplx_out, conjA=0, conjB=0):
+    L = hip.load()
+    M, K = Ar.shape
+    N = Br.shape[1]
+    fp = C.POINTER(C.c_float)
+    L.gpe_gemm_test.argtypes = [fp, fp, fp, fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.gpe_gemm_test.restype = C.c_int
+
+    def p(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(fp)
+
+    keep = [np.ascontiguousarray(x, dtype=np.float32) if x is not None else None for x in (Ar, Ai, Br, Bi)]
+    Cr = np.zeros((M, N), dtype=np.float32)
+    Ci = np.zeros((M, N), dtype=np.float32) if cplx_out else None
+    rc = L.gpe_gemm_test(*(p(x) for x in keep), p(Cr), p(Ci), M, N, K, conjA, conjB)
+    assert rc == 0
+    return Cr, Ci
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (16, 32, 16), (100, 70, 45), (9, 288, 16), (1, 5, 130)])
+def test_real_gemm(M, N, K):
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    Cr, _ = run(A, None, B, None, False)
+    ref = bf16(A).astype(np.float64) @ bf16(B).astype(np.float64)
+    assert np.abs(Cr - ref).max() < 1e-4 * np.abs(ref).max() + 1e-6
+    # and against the unrounded product: BF16 operand rounding (~2^-9 relative)
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.linalg.norm(Cr - exact) < 1e-2 * np.linalg.norm(exact)
+
+
+@pytest.mark.parametrize("conjA,conjB", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (33, 47, 20)])
+def test_complex_gemm(M, N, K, conjA, conjB):
+    rng = np.random.default_rng(7 * M + N)
+    A = (rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))).astype(np.complex64)
+    B = (rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))).astype(np.complex64)
+    Cr, Ci = run(A.real, A.imag, B.real, B.imag, True, conjA, conjB)
+    Ab = bf16(A.real) + 1j * bf16(A.imag)
+    Bb = bf16(B.real) + 1j * bf16(B.imag)
+    ref = (np.conj(Ab) if conjA else Ab).astype(np.complex128) @ (np.conj(Bb) if conjB else Bb).astype(np.complex128)
+    got = Cr + 1j * Ci
+    assert np.abs(got - ref).max() < 1e-4 * np.abs(ref).max()
+
+
+def test_mixed_real_complex_and_real_part_only():
+    rng = np.random.default_rng(3)
+    A = (rng.standard_normal((40, 24)) + 1j * rng.standard_normal((40, 24))).astype(np.complex64)
+    B = rng.standard_normal((24, 36)).astype(np.float32)
+    Cr, Ci = run(A.real, A.imag, B, None, True)
+    ref = (bf16(A.real) + 1j * bf16(A.imag)).astype(np.complex128) @ bf16(B).astype(np.float64)
+    assert np.abs(Cr + 1j * Ci - ref).max() < 1e-4 * np.abs(ref).max()
+    Bc = (rng.standard_normal((24, 36)) + 1j * rng.standard_normal((24, 36))).astype(np.complex64)
+    Cr, _ = run(A.real, A.imag, Bc.real, Bc.imag, False)
+    ref = ((bf16(A.real) + 1j * bf16(A.imag)).astype(np.complex128)
+           @ (bf16(Bc.real) + 1j * bf16(Bc.imag)).astype(np.complex128)).real
+    assert np.abs(Cr - ref).max() < 1e-4 * np.abs(ref).max()

@@ -148,28 +148,30 @@ def test_potential_half_step(oracle, shape, prec):
 
 @pytest.mark.parametrize("shape", SHAPES)
 def test_rt_fused_steps(oracle, shape, prec):
-    p, dev, orc = make_pair(oracle, shape, prec)
+    # FP32 runs at 500 atoms: ~0.05 atoms per grid point as in the 128x128x32,
+    # 2e4-atom configuration.  (At 2e4 atoms on these small grids the
+    # nonlinear phase reaches tens of radians per step and the FP32 storage of
+    # psi alone limits a step to ~1e-5.)
+    p, dev, orc = make_pair(oracle, shape, prec, n_atoms=2e4 if prec == hip.GPE_FP64 else 500.0)
     # start RT from a Phi computed on the loaded state (FS hand-over)
     orc.calc_fi3d()
     dev.import_fi3d(orc.fi3d.copy())
     dev.rt_steps(1)
     orc.rt_steps(0, 1)
-    # one step: FP32 error is the FFT round-off floor (~2-5e-7)
+    # one step: FP32 error is the FFT round-off floor (~1-3e-7)
     assert rel(dev.export_psi(), orc.psi) < (TOL_PSI[prec] if prec == hip.GPE_FP64 else 1e-6)
     dev.rt_steps(19)
     orc.rt_steps(1, 19)
-    # 20 steps: in these small, strongly interacting double wells the real-time
-    # dynamics amplify any perturbation ~20x over 20 steps (the FP64 path shows
-    # the same growth, 5e-15 -> 1e-13), so the FP32 bound is 3e-4 here.
-    tol = TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-4
+    # 20 steps: round-off accumulates (and double-well dynamics amplify it)
+    tol = TOL_PSI[prec] if prec == hip.GPE_FP64 else 2e-5
     assert rel(dev.export_psi(), orc.psi) < tol
-    assert rel(dev.export_fi3d(), orc.fi3d) < (10 * TOL_PHI[prec] if prec == hip.GPE_FP64 else 3e-4)
+    assert rel(dev.export_fi3d(), orc.fi3d) < (10 * TOL_PHI[prec] if prec == hip.GPE_FP64 else 2e-5)
 
 
 @pytest.mark.parametrize("shape", SHAPES)
 def test_rt_fused_single_well_fp32(oracle, shape):
     """Single well (HARMONIC, GAUSS): FP32 real time stays within 2e-5 over 20 steps."""
-    p, dev, orc = make_pair(oracle, shape, hip.GPE_FP32, pot="HARMONIC", init="GAUSS")
+    p, dev, orc = make_pair(oracle, shape, hip.GPE_FP32, pot="HARMONIC", init="GAUSS", n_atoms=500.0)
     orc.calc_fi3d()
     dev.import_fi3d(orc.fi3d.copy())
     dev.rt_steps(20)
@@ -263,11 +265,13 @@ def test_generic_runtime_size_kernels(oracle, shape, force, monkeypatch):
     on compiled shapes, run the run-time-size radix-4/2/3 engine."""
     monkeypatch.setenv("GPE_FORCE_GENERIC", force)
     for prec in (hip.GPE_FP64, hip.GPE_FP32):
-        p, dev, orc = make_pair(oracle, shape, prec, pot="HARMONIC", init="GAUSS")
+        p, dev, orc = make_pair(oracle, shape, prec, pot="HARMONIC", init="GAUSS", n_atoms=500.0)
         dev.it_steps(10)
         orc.it_steps(10)
         assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+        # real time from the oracle's state, so only the RT error is measured
         orc.calc_fi3d()
+        dev.load_buffer(orc.psi)
         dev.import_fi3d(orc.fi3d.copy())
         dev.rt_steps(3)
         orc.rt_steps(0, 3)
