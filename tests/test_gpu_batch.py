@@ -55,12 +55,14 @@ def test_batch_matches_single_and_oracle(oracle, prec):
 
 
 def test_batch_convergence_drops_instances():
-    setups = [setup_instance(*c) for c in CASES]
+    # a small box converges in a few hundred steps, the 64x64x16 one needs more
+    conv_cases = [((16, 16, 8), 1.4, 1e4), ((64, 64, 16), 1.5, 2e4), ((32, 24, 12), 1.3, 8e3)]
+    setups = [setup_instance(*c) for c in conv_cases]
     b = hip.Batch([s[0].to_gpe() for s in setups], hip.GPE_FP32)
     for d, (p, V, psi) in zip(b.instances, setups):
         d.load_pote(V)
         d.load_buffer(psi)
-    steps, e = b.it_until(2000, check_every=100, tol=1e-10)
+    steps, e = b.it_until(2000, check_every=100, tol=1e-6)
     assert np.all(steps % 100 == 0) and np.all(steps <= 2000) and np.all(steps >= 200)
     assert len(set(steps.tolist())) > 1  # instances converge at different steps
     # each converged instance satisfies the criterion at its last check
@@ -69,5 +71,5 @@ def test_batch_convergence_drops_instances():
             e1 = d.energy()["e_total"]
             d.it_steps(100)
             e2 = d.energy()["e_total"]
-            assert abs(e2 - e1) < 1e-10 * abs(e2) * 3
+            assert abs(e2 - e1) < 1e-6 * abs(e2) * 1.5
     b.close()
