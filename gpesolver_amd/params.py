@@ -197,8 +197,8 @@ def _gauss_sum(p, centers, sig, use_z_center):
 def wavefunction(strategy: str, p: PhysicalParameters, n_gauss_max: int | None = None, drops=None,
                  sigma_cells: float = 0.0) -> np.ndarray:
     """Initial state on the interior points, boundary zero (initializer.cpp:46-219).
-    sigma_cells > 0 sets the GAUSS width to that many cells per axis (the
-    reference uses L/10)."""
+    sigma_cells > 0 sets the GAUSS / MULTIPLE_GAUSS width to that many cells
+    per axis (the reference uses L/10 and L/20)."""
     nx, ny, nz = p.nx, p.ny, p.nz
     Lx, Ly, Lz = nx * p.dx, ny * p.dy, nz * p.dz
     if strategy == "COS":
@@ -216,6 +216,9 @@ def wavefunction(strategy: str, p: PhysicalParameters, n_gauss_max: int | None =
         nmax = n_gauss_max if n_gauss_max is not None else p.n_gauss_max
         if nmax <= 0:
             raise ValueError("Number of Gaussian maxima must be positive.")
+        sig = (Lx / 20.0, Ly / 20.0, Lz / 20.0)
+        if sigma_cells > 0:
+            sig = (sigma_cells * p.dx, sigma_cells * p.dy, sigma_cells * p.dz)
         centers = []
         for idx in range(nmax):
             cx = p.dd if idx % 2 else -p.dd
@@ -224,7 +227,7 @@ def wavefunction(strategy: str, p: PhysicalParameters, n_gauss_max: int | None =
             else:
                 cy = coords(ny, p.dy)[0] + (idx // 2 + 1) * (ny * p.dy) / (nmax // 2 + 1)
             centers.append((cx, cy, 0.0))
-        val = _gauss_sum(p, centers, (Lx / 20.0, Ly / 20.0, Lz / 20.0), False)
+        val = _gauss_sum(p, centers, sig, False)
     elif strategy == "SETUP_GAUSS":
         bx, by, bz = drops if drops is not None else p.bec_droplets
         centers = [(-Lx / 2 + Lx / (bx + 1) * (a + 1), -Ly / 2 + Ly / (by + 1) * (b + 1), -Lz / 2 + Lz / (bz + 1) * (c + 1))

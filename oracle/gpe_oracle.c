@@ -163,8 +163,14 @@ int orc_init_wavefunction(int strategy, const orc_params *p, int n_gauss_max, in
                     val = exp(-0.5 * (x * x) / (sx * sx)) * exp(-0.5 * (y * y) / (sy * sy)) *
                           exp(-0.5 * (z * z) / (sz * sz));
                 } else {
-                    /* initializer.cpp:133-159 and 192-218: sigma = L/20 per axis */
+                    /* initializer.cpp:133-159 and 192-218: sigma = L/20 per axis
+                     * (sigma_cells > 0 overrides it for MULTIPLE_GAUSS) */
                     double sx = (nx * p->dx) / 20., sy = (ny * p->dy) / 20., sz = (nz * p->dz) / 20.;
+                    if (sigma_cells > 0 && strategy == ORC_INIT_MULTIPLE_GAUSS) {
+                        sx = sigma_cells * p->dx;
+                        sy = sigma_cells * p->dy;
+                        sz = sigma_cells * p->dz;
+                    }
                     for (int m = 0; m < nmax; m++) {
                         double xc = x - cx[m], yc = y - cy[m];
                         double zc = (strategy == ORC_INIT_SETUP_GAUSS) ? z - cz[m] : z;
