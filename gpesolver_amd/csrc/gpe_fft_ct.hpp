@@ -185,6 +185,11 @@ constexpr bool supported_length(int L) {
 // One Stockham step: radix R, sub-transform size NS, on `nlines` lines of
 // length L in LDS (element e of line l at buf[l*ls + e*es]).  MAXL bounds the
 // number of lines (compile time) so each thread's butterflies fit in registers.
+// Butterfly w works on line w % nlines, so the lanes of a wave touch
+// neighbouring lines: with ls == 1 or ls odd their LDS addresses fall in
+// distinct banks for every read and write.  (The butterfly-major order puts
+// the lanes' writes R*es apart, which is a multiple of the bank count for
+// the padded plane layouts: a 16-way conflict on the first step.)
 template <typename T, int L, int R, int NS, int NT, int MAXL, int SIGN>
 __device__ __forceinline__ void step(cplx<T>* buf, int nlines, int es, int ls, const cplx<T>* __restrict__ tw) {
     constexpr int NB = L / R;
@@ -196,8 +201,8 @@ __device__ __forceinline__ void step(cplx<T>* buf, int nlines, int es, int ls, c
     for (int q = 0; q < MAXQ; ++q) {
         const int w = threadIdx.x + q * NT;
         if (w < total) {
-            const int line = w / NB;
-            const int j = w - line * NB;
+            const int j = w / nlines;
+            const int line = w - j * nlines;
             const cplx<T>* base = buf + line * ls + j * es;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = base[r * NB * es];
@@ -208,8 +213,8 @@ __device__ __forceinline__ void step(cplx<T>* buf, int nlines, int es, int ls, c
     for (int q = 0; q < MAXQ; ++q) {
         const int w = threadIdx.x + q * NT;
         if (w < total) {
-            const int line = w / NB;
-            const int j = w - line * NB;
+            const int j = w / nlines;
+            const int line = w - j * nlines;
             const int k = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll

@@ -41,7 +41,14 @@ using gpe::LinePlan;
 
 namespace {
 
-constexpr int kPlaneThreads = 512;
+// Threads of a plane block.  The compiled FP32 shapes run 16 waves per
+// plane (more loads in flight per CU, and they fit in 128 VGPRs); FP64, the
+// run-time-size kernel and the batched kernel (which holds every shape) run
+// 8 waves, since they need more than 128 VGPRs.
+template <typename T, bool COMPILED>
+struct PlaneThreads {
+    static constexpr int value = (sizeof(T) == 4 && COMPILED) ? 1024 : 512;
+};
 constexpr int kLineThreads = 256;
 constexpr int kMaxPlanePts = 8192;  // ny*nz bound of the single-block plane path
 constexpr int kLineMaxE = 4096;     // complex elements of one x-line tile
@@ -91,6 +98,7 @@ struct PlaneArgs {
     const double* part_in;
     double* part_out;
     const cplx<T>* tw;
+    int tw_n;           // entries of the twiddle table (staged in LDS by the compiled shapes)
     const cplx<T>* ez;  // exp(i fz(k)) / nz
     const cplx<T>* ey;  // exp(i fy(j)) / ny
     LinePlan pz2, py2, pz, py;
@@ -158,11 +166,15 @@ __device__ double block_sum(double v, double* red) {
 }
 
 // Sum of the per-plane partials, same order in every block.
+// Lane l of the first wave adds p[l], p[l+64], ... and a fixed shuffle tree
+// combines the lanes.
 __device__ double sum_partials(const double* p, int n, double* red) {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
         double s = 0.0;
-        for (int i = 0; i < n; ++i) s += p[i];
-        red[0] = s;
+        for (int t = threadIdx.x; t < n; t += 64) s += p[t];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (threadIdx.x == 0) red[0] = s;
     }
     __syncthreads();
     double r = red[0];
@@ -175,6 +187,7 @@ __device__ double sum_partials(const double* p, int n, double* red) {
 // sizes and the line FFTs, either at run time (any 2^a 3^b grid, generic
 // radix-4/2/3 engine) or as compile-time constants (large-radix engine).
 struct PlanePolGeneric {
+    static constexpr bool kStage = false;  // tables stay in global memory
     template <class A>
     __device__ static int ny(const A& a) { return a.ny; }
     template <class A>
@@ -183,26 +196,30 @@ struct PlanePolGeneric {
     static constexpr int ppt() { return kMaxPlanePts / NT; }
     template <typename T, int NT>
     static constexpr int rq() { return (Lim<T>::maxe / 2 + NT - 1) / NT; }
+    template <int NT>
+    static constexpr int rq2() { return 1; }  // unused: no staged S1 load
     template <typename T, int NT, int S>
-    __device__ static void fft_y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py2, nl, es, ls, S, a.tw);
+    __device__ static void fft_y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py2, nl, es, ls, S, tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz2, nl, es, ls, S, a.tw);
+    __device__ static void fft_z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz2, nl, es, ls, S, tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py, nl, es, ls, S, a.tw);
+    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py, nl, es, ls, S, tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz, nl, es, ls, S, a.tw);
+    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz, nl, es, ls, S, tw);
     }
 };
 
 template <int NY, int NZ>
 struct PlanePolCT {
+    // twiddle / kinetic tables and the psi, V loads are staged at block start
+    static constexpr bool kStage = true;
     template <class A>
     __device__ static constexpr int ny(const A&) { return NY; }
     template <class A>
@@ -211,21 +228,23 @@ struct PlanePolCT {
     static constexpr int ppt() { return (NY * NZ + NT - 1) / NT; }
     template <typename T, int NT>
     static constexpr int rq() { return (NY * (NZ + 1) + NT - 1) / NT; }
+    template <int NT>
+    static constexpr int rq2() { return (2 * NY * (NZ + 1) + NT - 1) / NT; }
     template <typename T, int NT, int S>
-    __device__ static void fft_y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::ct::fft<T, 2 * NY, NT, NZ + 1, S>(b, nl, es, ls, a.tw + a.py2.tw);
+    __device__ static void fft_y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::ct::fft<T, 2 * NY, NT, NZ + 1, S>(b, nl, es, ls, tw + a.py2.tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::ct::fft<T, 2 * NZ, NT, NY, S>(b, nl, es, ls, a.tw + a.pz2.tw);
+    __device__ static void fft_z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::ct::fft<T, 2 * NZ, NT, NY, S>(b, nl, es, ls, tw + a.pz2.tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::ct::fft<T, NY, NT, NZ, S>(b, nl, es, ls, a.tw + a.py.tw);
+    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::ct::fft<T, NY, NT, NZ, S>(b, nl, es, ls, tw + a.py.tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
-        gpe::ct::fft<T, NZ, NT, NY, S>(b, nl, es, ls, a.tw + a.pz.tw);
+    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+        gpe::ct::fft<T, NZ, NT, NY, S>(b, nl, es, ls, tw + a.pz.tw);
     }
 };
 
@@ -257,11 +276,15 @@ struct LinePolCT {
 };
 
 // ---------------------------------------------------------------------------
-// Plane kernel.  Dynamic LDS: max(2ny*(nz+1), ny*nz) complex T.
+// Plane kernel.  Dynamic LDS: the plane tile, max(2ny*(nz+1), ny*(2nz+1))
+// complex T, followed for the compiled shapes by the staged tables: the
+// twiddle table (tw_n entries), ez (nz) and ey (ny).
 template <typename T, int NT, class Pol>
 __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, unsigned char* lds_raw, double* red) {
     constexpr int PPT = Pol::template ppt<NT>();  // points per thread
     constexpr int RQ = Pol::template rq<T, NT>();
+    constexpr bool kStage = Pol::kStage;
+    constexpr int PRE = kStage ? PPT : 1;
     cplx<T>* P = reinterpret_cast<cplx<T>*>(lds_raw);
     const int nx = a.nx, ny = Pol::ny(a), nz = Pol::nz(a);
     const int nzh = nz + 1, nz2 = 2 * nz, ny2 = 2 * ny;
@@ -272,6 +295,49 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
     const int npl = ny * nz;
     const int mode = a.mode;
     const size_t plane_off = (size_t)i * npl;
+    const bool is_it = (mode & M_IT) != 0;
+    const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
+    const bool need_v = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL)) != 0;
+
+    // ---- 0. staging (compiled shapes): tables into LDS, and the loads of
+    // this plane's psi and V issued now so that their latency overlaps the
+    // Poisson finish.  Every FFT below follows a barrier, which publishes
+    // the tables.
+    const cplx<T>* tw = a.tw;
+    const cplx<T>* ez = a.ez;
+    const cplx<T>* ey = a.ey;
+    cplx<T> pre[PRE];
+    T vpre[PRE];
+    if constexpr (kStage) {
+        cplx<T>* TW = P + max(ny2 * nzh, ny * zs);
+        for (int m = threadIdx.x; m < a.tw_n; m += NT) TW[m] = a.tw[m];
+        tw = TW;
+        if (mode & M_KIN) {
+            cplx<T>* EZ = TW + a.tw_n;
+            cplx<T>* EY = EZ + nz;
+            for (int m = threadIdx.x; m < nz; m += NT) EZ[m] = a.ez[m];
+            for (int m = threadIdx.x; m < ny; m += NT) EY[m] = a.ey[m];
+            ez = EZ;
+            ey = EY;
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            pre[q] = gpe::mk(T(0), T(0));
+            vpre[q] = T(0);
+            if (p < npl) {
+                if (have_psi) {
+                    if (!is_it && (mode & M_IN_TK)) {
+                        const int j = p / nz, k = p - j * nz;
+                        pre[q] = a.tk[((size_t)j * nx + i) * nz + k];
+                    } else {
+                        pre[q] = a.in[plane_off + p];
+                    }
+                }
+                if (need_v) vpre[q] = a.V[plane_off + p];
+            }
+        }
+    }
 
     T phi_r[PPT];
     T rho_r[PPT];
@@ -282,12 +348,30 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
     // ---- 1. Phi for this plane -------------------------------------------
     if (mode & M_PINV) {
         // load S1[ky][i][kz] -> P[ky][kz]
-        for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
-            const int ky = e / nzh, kz = e - ky * nzh;
-            P[e] = a.s1[((size_t)ky * nx + i) * nzh + kz];
+        if constexpr (kStage) {
+            constexpr int R2 = Pol::template rq2<NT>();
+            cplx<T> st[R2];
+#pragma unroll
+            for (int q = 0; q < R2; ++q) {
+                const int e = threadIdx.x + q * NT;
+                if (e < ny2 * nzh) {
+                    const int ky = e / nzh, kz = e - ky * nzh;
+                    st[q] = a.s1[((size_t)ky * nx + i) * nzh + kz];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < R2; ++q) {
+                const int e = threadIdx.x + q * NT;
+                if (e < ny2 * nzh) P[e] = st[q];
+            }
+        } else {
+            for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
+                const int ky = e / nzh, kz = e - ky * nzh;
+                P[e] = a.s1[((size_t)ky * nx + i) * nzh + kz];
+            }
         }
         __syncthreads();
-        Pol::template fft_y2<T, NT, +1>(P, nzh, nzh, 1, a);
+        Pol::template fft_y2<T, NT, +1>(P, nzh, nzh, 1, a, tw);
         // rows j < ny -> Z[j][0..2nz) with Hermitian extension, via registers
         cplx<T> tmp[RQ];
 #pragma unroll
@@ -306,7 +390,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
             }
         }
         __syncthreads();
-        Pol::template fft_z2<T, NT, +1>(P, ny, 1, zs, a);
+        Pol::template fft_z2<T, NT, +1>(P, ny, 1, zs, a, tw);
         const T sc = (T)(a.pinv_scale * s_in * s_in);
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
@@ -328,13 +412,20 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
 
     // ---- 2. psi update: IT stencil or RT phase ------------------------------
     double nacc = 0.0;
-    const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
     cplx<T>* Q = P;  // aliases P: P is free here
-    if (mode & M_IT) {
+    if (is_it) {
         const cplx<T>* in = a.in;
-        for (int p = threadIdx.x; p < npl; p += NT) {
-            cplx<T> v = in[plane_off + p];
-            Q[(p / nz) * qs + p % nz] = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+        if constexpr (kStage) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < npl) Q[(p / nz) * qs + p % nz] = gpe::mk((T)(pre[q].x * s_in), (T)(pre[q].y * s_in));
+            }
+        } else {
+            for (int p = threadIdx.x; p < npl; p += NT) {
+                cplx<T> v = in[plane_off + p];
+                Q[(p / nz) * qs + p % nz] = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+            }
         }
         __syncthreads();
         const T sv = (T)s_in;
@@ -352,7 +443,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
                     const cplx<T> c = Q[qi];
                     const cplx<T> xm = in[plane_off - npl + p], xp = in[plane_off + npl + p];
                     const cplx<T> ym = Q[qi - qs], yp = Q[qi + qs], zm = Q[qi - 1], zp = Q[qi + 1];
-                    const T v = a.V[plane_off + p] + cdd * phi_r[q];
+                    const T v = (kStage ? vpre[q < PRE ? q : 0] : a.V[plane_off + p]) + cdd * phi_r[q];
                     cplx<T> c1;
                     c1.x = cx * (xm.x * sv + xp.x * sv - T(2) * c.x) + cy * (ym.x + yp.x - T(2) * c.x) +
                            cz * (zm.x + zp.x - T(2) * c.x) + c.x * v;
@@ -376,8 +467,18 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         }
         __syncthreads();
     } else if (have_psi) {
-        // load psi plane into Q
-        if (mode & M_IN_TK) {
+        // psi plane into Q
+        if constexpr (kStage) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < npl) {
+                    cplx<T> v = pre[q];
+                    if (!(mode & M_IN_TK) && (mode & M_NORM_IN)) v = gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+                    Q[(p / nz) * qs + p % nz] = v;
+                }
+            }
+        } else if (mode & M_IN_TK) {
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
                 Q[j * qs + k] = a.tk[((size_t)j * nx + i) * nz + k];
@@ -415,8 +516,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
                         // the sincos of the bulk precision, so a large phase
                         // (dense states, long steps) keeps its absolute accuracy
                         const double dn = (double)nrm;
-                        const double veff = a.cv * ((double)a.V[plane_off + p] + a.cdd * (double)phi_r[q]) +
-                                            pg * dn + pgm * dn * sqrt(dn);
+                        const double vv = kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
+                        const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * dn * sqrt(dn);
                         double th = -a.hdt * veff;
                         th -= 6.283185307179586 * rint(th * 0.15915494309189535);
                         T sn, cs;
@@ -436,21 +537,21 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         }
         if (mode & M_KIN) {
             // z propagation: rows j, length nz
-            Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a);
+            Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a, tw);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int qi = (p / nz) * qs + p % nz;
-                Q[qi] = gpe::cmul(Q[qi], a.ez[p % nz]);
+                Q[qi] = gpe::cmul(Q[qi], ez[p % nz]);
             }
             __syncthreads();
-            Pol::template fft_z<T, NT, +1>(Q, ny, 1, qs, a);
+            Pol::template fft_z<T, NT, +1>(Q, ny, 1, qs, a, tw);
             // y propagation: columns k, length ny
-            Pol::template fft_y<T, NT, -1>(Q, nz, qs, 1, a);
+            Pol::template fft_y<T, NT, -1>(Q, nz, qs, 1, a, tw);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int qi = (p / nz) * qs + p % nz;
-                Q[qi] = gpe::cmul(Q[qi], a.ey[p / nz]);
+                Q[qi] = gpe::cmul(Q[qi], ey[p / nz]);
             }
             __syncthreads();
-            Pol::template fft_y<T, NT, +1>(Q, nz, qs, 1, a);
+            Pol::template fft_y<T, NT, +1>(Q, nz, qs, 1, a, tw);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
                 a.tk[((size_t)j * nx + i) * nz + k] = Q[j * qs + k];
@@ -477,7 +578,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
             }
         }
         __syncthreads();
-        Pol::template fft_z2<T, NT, -1>(P, ny, 1, zs, a);
+        Pol::template fft_z2<T, NT, -1>(P, ny, 1, zs, a, tw);
         cplx<T> tmp[RQ];
 #pragma unroll
         for (int q = 0; q < RQ; ++q) {
@@ -495,7 +596,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         }
         for (int e = ny * nzh + threadIdx.x; e < ny2 * nzh; e += NT) P[e] = gpe::mk(T(0), T(0));
         __syncthreads();
-        Pol::template fft_y2<T, NT, -1>(P, nzh, nzh, 1, a);
+        Pol::template fft_y2<T, NT, -1>(P, nzh, nzh, 1, a, tw);
         for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
             const int ky = e / nzh, kz = e - ky * nzh;
             a.s1[((size_t)ky * nx + i) * nzh + kz] = P[e];
@@ -577,6 +678,119 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
     line_body<T, NT, Pol>(a, blockIdx.x, lds_raw);
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident x pass (power-of-two nx).  Each x line of length
+// L = A*B is held by B threads, A values each (A = 16):
+//   x[B a + b] --DFT_A over a--> Y[b][k1] * W_L^{b k1}  --LDS exchange-->
+//   thread t: rows k1 = t + B r --DFT_B over b--> X[k1 + A k2]
+// the multiplier (Vdd or the kinetic phase) is applied there, and the
+// inverse transform runs the same steps backwards from the same registers,
+// so a forward / multiply / inverse line costs two LDS exchanges.  Global
+// loads and stores go straight to / from registers; lines are the fastest
+// lane index, so neighbouring lanes touch neighbouring kz / k (contiguous).
+template <typename T, int L, int LPB, bool POISSON>
+__device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>* E, const cplx<T>* tw) {
+    constexpr int A = 16, B = L / A, R = A / B;
+    constexpr int LS = LPB + 1;  // padded row stride of the exchange area
+    static_assert(L % A == 0 && A % B == 0, "line split");
+    const int nx = a.nx, nzh = a.nz + 1, nz = a.nz;
+    const int nlines = POISSON ? 2 * a.ny * nzh : a.ny * nz;
+    const int ll = threadIdx.x % LPB, b = threadIdx.x / LPB;
+    const int line = blk * LPB + ll;
+    const bool valid = line < nlines;
+    cplx<T>* base;
+    int estride, ky = 0, kz = 0;
+    if (POISSON) {
+        ky = valid ? line / nzh : 0;
+        kz = valid ? line - ky * nzh : 0;
+        base = a.s1 + (size_t)ky * nx * nzh + kz;
+        estride = nzh;
+    } else {
+        const int j = valid ? line / nz : 0;
+        const int k = valid ? line - j * nz : 0;
+        base = a.tk + (size_t)j * nx * nz + k;
+        estride = nz;
+    }
+    // forward, stage 1
+    constexpr int AL = POISSON ? A / 2 : A;  // Poisson: x >= nx is the zero padding
+    cplx<T> v[A];
+#pragma unroll
+    for (int q = 0; q < A; ++q) {
+        v[q] = gpe::mk(T(0), T(0));
+        if (q < AL && valid) v[q] = base[(size_t)(B * q + b) * estride];
+    }
+    gpe::ct::Dft<T, A, -1>::run(v);
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmul(v[k1], tw[(b * k1) % L]);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = v[k1];
+    __syncthreads();
+    // forward stage 2, multiply, inverse stage 1 -- rows k1 = b + B r
+    cplx<T> w[R][B];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k1 = b + B * r;
+#pragma unroll
+        for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * LS + ll];
+        gpe::ct::Dft<T, B, -1>::run(w[r]);
+        if (POISSON) {
+            const T ky2 = (T)a.ky2[ky], kz2 = (T)a.kz2[kz];
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) {
+                const T k2s = (T)a.kx2[k1 + A * k2] + ky2 + kz2;
+                const T vd = (k2s > T(1e-30)) ? kz2 / k2s : T(0);
+                w[r][k2] = gpe::cscale(w[r][k2], vd);
+            }
+        } else {
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::cmul(w[r][k2], a.ex[k1 + A * k2]);
+        }
+        gpe::ct::Dft<T, B, +1>::run(w[r]);
+#pragma unroll
+        for (int bb = 1; bb < B; ++bb) w[r][bb] = gpe::cmulc(w[r][bb], tw[(bb * k1) % L]);
+    }
+    __syncthreads();  // every stage-2 read of E is done before it is rewritten
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k1 = b + B * r;
+#pragma unroll
+        for (int bb = 0; bb < B; ++bb) E[(k1 * B + bb) * LS + ll] = w[r][bb];
+    }
+    __syncthreads();
+    // inverse stage 2: thread b gathers Z[k1][b] for every k1
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * LS + ll];
+    gpe::ct::Dft<T, A, +1>::run(v);
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < AL; ++q) base[(size_t)(B * q + b) * estride] = v[q];
+    }
+}
+
+template <int NX>
+struct XReg {
+    static constexpr int LP = 2 * NX, LK = NX;          // Poisson / kinetic line lengths
+    static constexpr int LPB_P = 256 / (LP / 16);       // lines per 256-thread block
+    static constexpr int LPB_K = 256 / (LK / 16);
+};
+
+template <typename T, int NX>
+__global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_p) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
+    cplx<T>* tws = E + 16 * 256 + 256;  // exchange area: L * (LPB + 1) <= 4096 + 256 elements
+    using X = XReg<NX>;
+    if ((int)blockIdx.x < nblk_p) {
+        for (int m = threadIdx.x; m < X::LP; m += 256) tws[m] = a.tw[a.px2.tw + m];
+        __syncthreads();
+        xline_reg<T, X::LP, X::LPB_P, true>(a, blockIdx.x, E, tws);
+    } else {
+        for (int m = threadIdx.x; m < X::LK; m += 256) tws[m] = a.tw[a.px.tw + m];
+        __syncthreads();
+        xline_reg<T, X::LK, X::LPB_K, false>(a, blockIdx.x - nblk_p, E, tws);
+    }
+}
+
 // Batched launches: one block per work item (instance, plane) / (instance,
 // tile); each instance brings its own argument block and may have its own grid.
 template <typename T, int NT>
@@ -623,13 +837,7 @@ __global__ void __launch_bounds__(NT) k_xpass_mixed(const LineArgs<T>* __restric
 template <typename T>
 __global__ void k_scale(cplx<T>* psi, size_t n, const double* part, int np) {
     __shared__ double sred[2];
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int t = 0; t < np; ++t) s += part[t];
-        sred[0] = 1.0 / sqrt(s);
-    }
-    __syncthreads();
-    const T sc = (T)sred[0];
+    const T sc = (T)(1.0 / sqrt(sum_partials(part, np, sred)));
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         cplx<T> v = psi[e];
         psi[e] = gpe::mk(v.x * sc, v.y * sc);
@@ -817,10 +1025,13 @@ struct gpe_handle {
     int pcur = 0;
     bool norm_pending = false;  // true: normalised psi = store[cur]/sqrt(sum part[pcur])
     bool force_generic = false; // GPE_FORCE_GENERIC=1: run-time-size kernels only
+    bool line_reg = true;       // register-resident x pass (GPE_LINE_LDS=1 disables)
     LinePlan px, py, pz, px2, py2, pz2;
     // launch shapes
     int cp = 0, ck = 0;
     size_t plane_lds = 0, line_lds = 0;
+    size_t plane_stage = 0;  // extra LDS of the compiled plane shapes (staged tables)
+    int tw_n = 0;            // entries of the twiddle table
     // scaled constants
     double sqdv = 1.0;
     double ggp11, gamma;
@@ -931,6 +1142,7 @@ static int setup_tables(gpe_handle* h) {
             tw.push_back(c);
         }
     }
+    h->tw_n = (int)tw.size();
     GPE_CHECK(hipMalloc(&h->tw, tw.size() * sizeof(cplx<T>)));
     GPE_CHECK(hipMemcpy(h->tw, tw.data(), tw.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
     // kinetic phase tables (fftw_split_solver.cpp:11-42, factorised per axis)
@@ -986,6 +1198,7 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.V = (const T*)h->V;
     a.phi = (T*)h->phi;
     a.tw = (const cplx<T>*)h->tw;
+    a.tw_n = h->tw_n;
     a.ez = (const cplx<T>*)h->ez;
     a.ey = (const cplx<T>*)h->ey;
     a.pz2 = h->pz2;
@@ -1034,10 +1247,22 @@ static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
     return a;
 }
 
+// The plane shape has a compiled kernel (PlanePolCT) that the launches use.
+static bool plane_compiled(const gpe_handle* h) {
+    if (h->force_generic) return false;
+    const int ny = h->p.ny, nz = h->p.nz;
+#define GPE_PL(NY, NZ) \
+    if (ny == NY && nz == NZ) return true;
+    GPE_PLANE_SHAPES(GPE_PL)
+#undef GPE_PL
+    return false;
+}
+
 template <typename T, class Pol>
 static int launch_plane_pol(gpe_handle* h, const PlaneArgs<T>& a) {
-    hipLaunchKernelGGL((k_plane<T, kPlaneThreads, Pol>), dim3(h->p.nx), dim3(kPlaneThreads), h->plane_lds, h->stream,
-                       a);
+    constexpr int NT = PlaneThreads<T, Pol::kStage>::value;
+    const size_t lds = h->plane_lds + (Pol::kStage ? h->plane_stage : 0);
+    hipLaunchKernelGGL((k_plane<T, NT, Pol>), dim3(h->p.nx), dim3(NT), lds, h->stream, a);
     GPE_CHECK(hipGetLastError());
     return 0;
 }
@@ -1062,12 +1287,28 @@ static int launch_line_pol(gpe_handle* h, const LineArgs<T>& a, int nblk) {
     return 0;
 }
 
+template <typename T, int NX>
+static int launch_line_reg(gpe_handle* h, const LineArgs<T>& a, bool poisson, bool kinetic) {
+    using X = XReg<NX>;
+    const int np = poisson ? (2 * h->p.ny * (h->p.nz + 1) + X::LPB_P - 1) / X::LPB_P : 0;
+    const int nk = kinetic ? (h->p.ny * h->p.nz + X::LPB_K - 1) / X::LPB_K : 0;
+    if (np + nk == 0) return 0;
+    const size_t lds = (size_t)(16 * 256 + 256 + 2 * NX) * sizeof(cplx<T>);
+    hipLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, a, np);
+    GPE_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 static int launch_line(gpe_handle* h, bool poisson, bool kinetic) {
     LineArgs<T> a = line_args<T>(h, poisson, kinetic);
     const int nblk = a.ntile_p + (kinetic ? h->p.ny * a.tiles_k : 0);
     if (nblk == 0) return 0;
     const int nx = h->p.nx;
+    if (!h->force_generic && h->line_reg) {
+        if (nx == 128) return launch_line_reg<T, 128>(h, a, poisson, kinetic);
+        if (nx == 64) return launch_line_reg<T, 64>(h, a, poisson, kinetic);
+    }
     if (!h->force_generic) {
 #define GPE_LN(NX) \
     if (nx == NX) return launch_line_pol<T, LinePolCT<NX>>(h, a, nblk);
@@ -1496,7 +1737,7 @@ static int batch_setup(gpe_batch* b) {
         la[q] = line_args<T>(h, true, false);
         for (int i = 0; i < h->p.nx; ++i) pw.push_back(make_int2((int)q, i));
         for (int t = 0; t < la[q].ntile_p; ++t) lw.push_back(make_int2((int)q, t));
-        b->plane_lds = std::max(b->plane_lds, h->plane_lds);
+        b->plane_lds = std::max(b->plane_lds, h->plane_lds + (plane_compiled(h) ? h->plane_stage : 0));
         b->line_lds = std::max(b->line_lds, h->line_lds);
     }
     void* old[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork};
@@ -1521,7 +1762,8 @@ static int batch_setup(gpe_batch* b) {
 template <typename T>
 static int batch_plane(gpe_batch* b, int slot) {
     const PlaneArgs<T>* args = (const PlaneArgs<T>*)b->d_pargs + (size_t)slot * b->active.size();
-    hipLaunchKernelGGL((k_plane_mixed<T, kPlaneThreads>), dim3(b->npwork), dim3(kPlaneThreads), b->plane_lds,
+    constexpr int NT = PlaneThreads<T, false>::value;
+    hipLaunchKernelGGL((k_plane_mixed<T, NT>), dim3(b->npwork), dim3(NT), b->plane_lds,
                        b->stream, args, (const int2*)b->d_pwork);
     BCHECK(hipGetLastError());
     return 0;
@@ -1655,6 +1897,8 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
         h->force_generic = fg && fg[0] == '1';
         const char* ng = getenv("GPE_NO_GRAPH");
         h->use_graphs = !(ng && ng[0] == '1');
+        const char* ll = getenv("GPE_LINE_LDS");
+        h->line_reg = !(ll && ll[0] == '1');
     }
     auto bad = [&](const std::string& m) {
         g_create_err = m;
@@ -1683,6 +1927,7 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     h->ck = std::min(p->nz, std::max(1, kLineMaxE / p->nx));
     h->ck = std::min(h->ck, 16);
     h->plane_lds = (size_t)std::max(2L * p->ny * (p->nz + 1), (long)p->ny * (2 * p->nz + 1)) * cs;
+    h->plane_stage = (size_t)(h->tw_n + p->ny + p->nz) * cs;
     h->line_lds = (size_t)std::max(2L * p->nx * (h->cp + 1), (long)p->nx * (h->ck + 1)) * cs;
     if (hipDeviceSynchronize() != hipSuccess) return bad("device synchronize failed after setup");
     *out = h;

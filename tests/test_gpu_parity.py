@@ -276,3 +276,24 @@ def test_generic_runtime_size_kernels(oracle, shape, force, monkeypatch):
         dev.rt_steps(3)
         orc.rt_steps(0, 3)
         assert rel(dev.export_psi(), orc.psi) < (TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-6)
+
+
+@pytest.mark.parametrize("shape", [(128, 16, 8), (64, 12, 6)])
+@pytest.mark.parametrize("line_lds", ["0", "1"])
+def test_x_pass_register_and_lds_kernels(oracle, shape, line_lds, monkeypatch):
+    """Power-of-two nx runs the register-resident x pass by default and the
+    LDS Stockham x pass with GPE_LINE_LDS=1; both against the oracle."""
+    monkeypatch.setenv("GPE_LINE_LDS", line_lds)
+    for prec in (hip.GPE_FP64, hip.GPE_FP32):
+        p, dev, orc = make_pair(oracle, shape, prec, pot="HARMONIC", init="GAUSS", n_atoms=500.0)
+        dev.calc_fi3d()
+        orc.calc_fi3d()
+        assert rel(dev.export_fi3d(), orc.fi3d) < TOL_PHI[prec]
+        dev.kinetic_step()
+        orc.kinetic_step()
+        assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
+        dev.load_buffer(orc.psi)
+        dev.import_fi3d(orc.fi3d.copy())
+        dev.rt_steps(4)
+        orc.rt_steps(0, 4)
+        assert rel(dev.export_psi(), orc.psi) < (TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-6)
