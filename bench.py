@@ -52,6 +52,25 @@ def build_case(precision):
     return p, dev, V, psi
 
 
+def calib_copy(nbytes: int):
+    """One hipMemcpy device-to-device of nbytes (a plain copy kernel that
+    reads and writes nbytes), for scaling the memory counters."""
+    import ctypes as C
+    hiprt = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hiprt.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hiprt.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        hiprt.hipMemset(a, 1, C.c_size_t(nbytes))
+        hiprt.hipDeviceSynchronize()
+        if hiprt.hipMemcpy(b, a, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+        hiprt.hipDeviceSynchronize()
+    finally:
+        hiprt.hipFree(a)
+        hiprt.hipFree(b)
+
+
 def model_bytes(nx, ny, nz, fp64=False):
     """Algorithmic bytes of one RT step: 112N + 20H (FP32), H = 2nx*2ny*(nz+1)."""
     N = nx * ny * nz
@@ -103,6 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64", action="store_true", help="also time the FP64 path")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="end with one device-to-device copy of this many MiB (the known-size "
+                         "reference dispatch for FETCH_SIZE / WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one solver instance on one GPU; --gpus must be 1 "
@@ -154,6 +176,9 @@ def main():
         d64.synchronize()
         extra["fp64_rt_steps_per_s"] = args.steps / (time.perf_counter() - t2)
         d64.close()
+
+    if args.calib_copy_mib > 0:
+        calib_copy(args.calib_copy_mib << 20)
 
     mb = model_bytes(nx, ny, nz)
     out = {

@@ -39,6 +39,14 @@
 using gpe::cplx;
 using gpe::LinePlan;
 
+// Streaming (nontemporal) store of one complex value.
+template <typename T>
+__device__ __forceinline__ void store_nt(cplx<T>* p, cplx<T> v) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<v2*>(p));
+}
+
 namespace {
 
 // Threads of a plane block.  The compiled FP32 shapes run 16 waves per
@@ -299,26 +307,27 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
     const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
     const bool need_v = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL)) != 0;
 
-    // ---- 0. staging (compiled shapes): tables into LDS, and the loads of
-    // this plane's psi and V issued now so that their latency overlaps the
-    // Poisson finish.  Every FFT below follows a barrier, which publishes
-    // the tables.
+    // ---- 0. staging (compiled shapes): the loads of this plane's S1
+    // column block, psi and V are issued first, then the tables go to LDS
+    // while they are in flight.  Every FFT below follows a barrier, which
+    // publishes the tables.
     const cplx<T>* tw = a.tw;
     const cplx<T>* ez = a.ez;
     const cplx<T>* ey = a.ey;
+    constexpr int R2 = kStage ? Pol::template rq2<NT>() : 1;
+    cplx<T> st[R2];  // S1 plane (Poisson finish input)
     cplx<T> pre[PRE];
     T vpre[PRE];
     if constexpr (kStage) {
-        cplx<T>* TW = P + max(ny2 * nzh, ny * zs);
-        for (int m = threadIdx.x; m < a.tw_n; m += NT) TW[m] = a.tw[m];
-        tw = TW;
-        if (mode & M_KIN) {
-            cplx<T>* EZ = TW + a.tw_n;
-            cplx<T>* EY = EZ + nz;
-            for (int m = threadIdx.x; m < nz; m += NT) EZ[m] = a.ez[m];
-            for (int m = threadIdx.x; m < ny; m += NT) EY[m] = a.ey[m];
-            ez = EZ;
-            ey = EY;
+        if (mode & M_PINV) {
+#pragma unroll
+            for (int q = 0; q < R2; ++q) {
+                const int e = threadIdx.x + q * NT;
+                if (e < ny2 * nzh) {
+                    const int ky = e / nzh, kz = e - ky * nzh;
+                    st[q] = a.s1[((size_t)ky * nx + i) * nzh + kz];
+                }
+            }
         }
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
@@ -337,6 +346,17 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
                 if (need_v) vpre[q] = a.V[plane_off + p];
             }
         }
+        cplx<T>* TW = P + max(ny2 * nzh, ny * zs);
+        for (int m = threadIdx.x; m < a.tw_n; m += NT) TW[m] = a.tw[m];
+        tw = TW;
+        if (mode & M_KIN) {
+            cplx<T>* EZ = TW + a.tw_n;
+            cplx<T>* EY = EZ + nz;
+            for (int m = threadIdx.x; m < nz; m += NT) EZ[m] = a.ez[m];
+            for (int m = threadIdx.x; m < ny; m += NT) EY[m] = a.ey[m];
+            ez = EZ;
+            ey = EY;
+        }
     }
 
     T phi_r[PPT];
@@ -349,16 +369,6 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
     if (mode & M_PINV) {
         // load S1[ky][i][kz] -> P[ky][kz]
         if constexpr (kStage) {
-            constexpr int R2 = Pol::template rq2<NT>();
-            cplx<T> st[R2];
-#pragma unroll
-            for (int q = 0; q < R2; ++q) {
-                const int e = threadIdx.x + q * NT;
-                if (e < ny2 * nzh) {
-                    const int ky = e / nzh, kz = e - ky * nzh;
-                    st[q] = a.s1[((size_t)ky * nx + i) * nzh + kz];
-                }
-            }
 #pragma unroll
             for (int q = 0; q < R2; ++q) {
                 const int e = threadIdx.x + q * NT;
@@ -689,7 +699,7 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
 // loads and stores go straight to / from registers; lines are the fastest
 // lane index, so neighbouring lanes touch neighbouring kz / k (contiguous).
 template <typename T, int L, int LPB, bool POISSON>
-__device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>* E, const cplx<T>* tw) {
+__device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>* E, cplx<T>* TB) {
     constexpr int A = 16, B = L / A, R = A / B;
     constexpr int LS = LPB + 1;  // padded row stride of the exchange area
     static_assert(L % A == 0 && A % B == 0, "line split");
@@ -711,7 +721,9 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
         base = a.tk + (size_t)j * nx * nz + k;
         estride = nz;
     }
-    // forward, stage 1
+    // issue the line loads first, then stage the tables while they are in
+    // flight: TB[0, L) = W_L^m, then (Poisson) kx^2 as T or (kinetic) the
+    // x phase table
     constexpr int AL = POISSON ? A / 2 : A;  // Poisson: x >= nx is the zero padding
     cplx<T> v[A];
 #pragma unroll
@@ -719,6 +731,23 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
         v[q] = gpe::mk(T(0), T(0));
         if (q < AL && valid) v[q] = base[(size_t)(B * q + b) * estride];
     }
+    T ky2 = T(0), kz2 = T(0);
+    if (POISSON) {
+        ky2 = (T)a.ky2[ky];
+        kz2 = (T)a.kz2[kz];
+    }
+    const cplx<T>* tw = TB;
+    T* kx2s = reinterpret_cast<T*>(TB + L);
+    const cplx<T>* exs = TB + L;
+    for (int m = threadIdx.x; m < L; m += 256) {
+        TB[m] = a.tw[(POISSON ? a.px2.tw : a.px.tw) + m];
+        if (POISSON)
+            kx2s[m] = (T)a.kx2[m];
+        else
+            TB[L + m] = a.ex[m];
+    }
+    __syncthreads();
+    // forward, stage 1
     gpe::ct::Dft<T, A, -1>::run(v);
 #pragma unroll
     for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmul(v[k1], tw[(b * k1) % L]);
@@ -734,16 +763,15 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
         for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * LS + ll];
         gpe::ct::Dft<T, B, -1>::run(w[r]);
         if (POISSON) {
-            const T ky2 = (T)a.ky2[ky], kz2 = (T)a.kz2[kz];
 #pragma unroll
             for (int k2 = 0; k2 < B; ++k2) {
-                const T k2s = (T)a.kx2[k1 + A * k2] + ky2 + kz2;
+                const T k2s = kx2s[k1 + A * k2] + ky2 + kz2;
                 const T vd = (k2s > T(1e-30)) ? kz2 / k2s : T(0);
                 w[r][k2] = gpe::cscale(w[r][k2], vd);
             }
         } else {
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::cmul(w[r][k2], a.ex[k1 + A * k2]);
+            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::cmul(w[r][k2], exs[k1 + A * k2]);
         }
         gpe::ct::Dft<T, B, +1>::run(w[r]);
 #pragma unroll
@@ -762,8 +790,10 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
     for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * LS + ll];
     gpe::ct::Dft<T, A, +1>::run(v);
     if (valid) {
+        // streaming stores: the next pass reads them from memory anyway, and
+        // lines that do not linger dirty in L2 shorten the end-of-kernel flush
 #pragma unroll
-        for (int q = 0; q < AL; ++q) base[(size_t)(B * q + b) * estride] = v[q];
+        for (int q = 0; q < AL; ++q) store_nt(&base[(size_t)(B * q + b) * estride], v[q]);
     }
 }
 
@@ -774,21 +804,17 @@ struct XReg {
     static constexpr int LPB_K = 256 / (LK / 16);
 };
 
+// LDS: exchange area (16 * 256 + 256 complex) + tables (2 * 2NX complex).
 template <typename T, int NX>
 __global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_p) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
-    cplx<T>* tws = E + 16 * 256 + 256;  // exchange area: L * (LPB + 1) <= 4096 + 256 elements
+    cplx<T>* TB = E + 16 * 256 + 256;  // exchange area: L * (LPB + 1) <= 4096 + 256 elements
     using X = XReg<NX>;
-    if ((int)blockIdx.x < nblk_p) {
-        for (int m = threadIdx.x; m < X::LP; m += 256) tws[m] = a.tw[a.px2.tw + m];
-        __syncthreads();
-        xline_reg<T, X::LP, X::LPB_P, true>(a, blockIdx.x, E, tws);
-    } else {
-        for (int m = threadIdx.x; m < X::LK; m += 256) tws[m] = a.tw[a.px.tw + m];
-        __syncthreads();
-        xline_reg<T, X::LK, X::LPB_K, false>(a, blockIdx.x - nblk_p, E, tws);
-    }
+    if ((int)blockIdx.x < nblk_p)
+        xline_reg<T, X::LP, X::LPB_P, true>(a, blockIdx.x, E, TB);
+    else
+        xline_reg<T, X::LK, X::LPB_K, false>(a, blockIdx.x - nblk_p, E, TB);
 }
 
 // Batched launches: one block per work item (instance, plane) / (instance,
@@ -1293,7 +1319,7 @@ static int launch_line_reg(gpe_handle* h, const LineArgs<T>& a, bool poisson, bo
     const int np = poisson ? (2 * h->p.ny * (h->p.nz + 1) + X::LPB_P - 1) / X::LPB_P : 0;
     const int nk = kinetic ? (h->p.ny * h->p.nz + X::LPB_K - 1) / X::LPB_K : 0;
     if (np + nk == 0) return 0;
-    const size_t lds = (size_t)(16 * 256 + 256 + 2 * NX) * sizeof(cplx<T>);
+    const size_t lds = (size_t)(16 * 256 + 256 + 4 * NX) * sizeof(cplx<T>);
     hipLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, a, np);
     GPE_CHECK(hipGetLastError());
     return 0;
