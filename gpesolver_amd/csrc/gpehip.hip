@@ -99,8 +99,11 @@ struct PlaneArgs {
     const cplx<T>* in;
     cplx<T>* out;
     const cplx<T>* psii;
-    cplx<T>* tk;
-    cplx<T>* s1;
+    cplx<T>* tk;        // kinetic work array, read (M_IN_TK)
+    cplx<T>* s1;        // Poisson spectrum, read (M_PINV)
+    cplx<T>* tko;       // written by M_KIN (== tk unless the buffers ping-pong)
+    cplx<T>* s1o;       // written by M_RHO_FWD (== s1 unless the buffers ping-pong)
+    int split;          // 1: two blocks per plane (grid 2 nx), see k_plane
     const T* V;
     T* phi;
     const double* part_in;
@@ -288,7 +291,8 @@ struct LinePolCT {
 // complex T, followed for the compiled shapes by the staged tables: the
 // twiddle table (tw_n entries), ez (nz) and ey (ny).
 template <typename T, int NT, class Pol>
-__device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, unsigned char* lds_raw, double* red) {
+__device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, const int mode, unsigned char* lds_raw,
+                                           double* red) {
     constexpr int PPT = Pol::template ppt<NT>();  // points per thread
     constexpr int RQ = Pol::template rq<T, NT>();
     constexpr bool kStage = Pol::kStage;
@@ -301,7 +305,6 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
     const int zs = nz2 + 1;  // z-line rows of the R2C/C2R stage
     const int qs = nz + 1;   // psi plane rows
     const int npl = ny * nz;
-    const int mode = a.mode;
     const size_t plane_off = (size_t)i * npl;
     const bool is_it = (mode & M_IT) != 0;
     const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
@@ -564,7 +567,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
             Pol::template fft_y<T, NT, +1>(Q, nz, qs, 1, a, tw);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
-                a.tk[((size_t)j * nx + i) * nz + k] = Q[j * qs + k];
+                a.tko[((size_t)j * nx + i) * nz + k] = Q[j * qs + k];
             }
         }
         __syncthreads();
@@ -609,7 +612,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, u
         Pol::template fft_y2<T, NT, -1>(P, nzh, nzh, 1, a, tw);
         for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
             const int ky = e / nzh, kz = e - ky * nzh;
-            a.s1[((size_t)ky * nx + i) * nzh + kz] = P[e];
+            a.s1o[((size_t)ky * nx + i) * nzh + kz] = P[e];
         }
     }
 }
@@ -675,11 +678,26 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
     }
 }
 
+// One block per x-plane, or (split) two: block i and block i + nx both
+// finish the Poisson solve and apply the phase of plane i; the first then
+// runs the kinetic z / y propagation (and any psi, fi3d or norm output),
+// the second starts the next Poisson solve.  The two halves of the plane
+// work run on different CUs, at the price of reading the plane twice; the
+// outputs go to the other buffer of a ping-pong pair (s1o, tko) so no
+// block overwrites an input another block may not have read yet.
 template <typename T, int NT, class Pol>
 __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[NT / 64 + 1];
-    plane_body<T, NT, Pol>(a, blockIdx.x, lds_raw, red);
+    int i = blockIdx.x, mode = a.mode;
+    if (a.split) {
+        if (i < a.nx)
+            mode &= ~M_RHO_FWD;
+        else
+            mode &= ~(M_KIN | M_OUT | M_PHI_STORE | M_NORM_OUT);
+        i = i < a.nx ? i : i - a.nx;
+    }
+    plane_body<T, NT, Pol>(a, i, mode, lds_raw, red);
 }
 
 template <typename T, int NT, class Pol>
@@ -829,13 +847,13 @@ __global__ void __launch_bounds__(NT) k_plane_mixed(const PlaneArgs<T>* __restri
     if (!a.generic) {
 #define GPE_PL(NY, NZ)                                                  \
     if (a.ny == NY && a.nz == NZ) {                                     \
-        plane_body<T, NT, PlanePolCT<NY, NZ>>(a, w.y, lds_raw, red);    \
+        plane_body<T, NT, PlanePolCT<NY, NZ>>(a, w.y, a.mode, lds_raw, red); \
         return;                                                         \
     }
         GPE_PLANE_SHAPES(GPE_PL)
 #undef GPE_PL
     }
-    plane_body<T, NT, PlanePolGeneric>(a, w.y, lds_raw, red);
+    plane_body<T, NT, PlanePolGeneric>(a, w.y, a.mode, lds_raw, red);
 }
 
 template <typename T, int NT>
@@ -1039,6 +1057,8 @@ struct gpe_handle {
     void* phi = nullptr;
     void* s1 = nullptr;
     void* tk = nullptr;
+    void* s1b = nullptr;  // second S1 / Tk of the split real-time plane pass (allocated on first use)
+    void* tkb = nullptr;
     void* tw = nullptr;
     void* ex = nullptr;
     void* ey = nullptr;
@@ -1052,6 +1072,7 @@ struct gpe_handle {
     bool norm_pending = false;  // true: normalised psi = store[cur]/sqrt(sum part[pcur])
     bool force_generic = false; // GPE_FORCE_GENERIC=1: run-time-size kernels only
     bool line_reg = true;       // register-resident x pass (GPE_LINE_LDS=1 disables)
+    bool plane_split = true;    // two blocks per plane in steady real time (GPE_NO_SPLIT=1 disables)
     LinePlan px, py, pz, px2, py2, pz2;
     // launch shapes
     int cp = 0, ck = 0;
@@ -1219,8 +1240,9 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.mode = mode;
     a.generic = h->force_generic ? 1 : 0;
     a.psii = (const cplx<T>*)h->psii;
-    a.tk = (cplx<T>*)h->tk;
-    a.s1 = (cplx<T>*)h->s1;
+    a.tk = a.tko = (cplx<T>*)h->tk;
+    a.s1 = a.s1o = (cplx<T>*)h->s1;
+    a.split = 0;
     a.V = (const T*)h->V;
     a.phi = (T*)h->phi;
     a.tw = (const cplx<T>*)h->tw;
@@ -1288,7 +1310,7 @@ template <typename T, class Pol>
 static int launch_plane_pol(gpe_handle* h, const PlaneArgs<T>& a) {
     constexpr int NT = PlaneThreads<T, Pol::kStage>::value;
     const size_t lds = h->plane_lds + (Pol::kStage ? h->plane_stage : 0);
-    hipLaunchKernelGGL((k_plane<T, NT, Pol>), dim3(h->p.nx), dim3(NT), lds, h->stream, a);
+    hipLaunchKernelGGL((k_plane<T, NT, Pol>), dim3(h->p.nx * (a.split ? 2 : 1)), dim3(NT), lds, h->stream, a);
     GPE_CHECK(hipGetLastError());
     return 0;
 }
@@ -1326,8 +1348,10 @@ static int launch_line_reg(gpe_handle* h, const LineArgs<T>& a, bool poisson, bo
 }
 
 template <typename T>
-static int launch_line(gpe_handle* h, bool poisson, bool kinetic) {
+static int launch_line(gpe_handle* h, bool poisson, bool kinetic, void* s1 = nullptr, void* tk = nullptr) {
     LineArgs<T> a = line_args<T>(h, poisson, kinetic);
+    if (s1) a.s1 = (cplx<T>*)s1;
+    if (tk) a.tk = (cplx<T>*)tk;
     const int nblk = a.ntile_p + (kinetic ? h->p.ny * a.tiles_k : 0);
     if (nblk == 0) return 0;
     const int nx = h->p.nx;
@@ -1466,6 +1490,7 @@ static int finish_timing(gpe_handle* h, size_t nev) {
 }
 
 constexpr int kGraphSteps = 64;  // steady-state steps per captured graph (even)
+static_assert(kGraphSteps % 2 == 0, "a replay must end on the S1 / Tk parity it started on");
 
 static int ensure_coef(gpe_handle* h, size_t steps) {
     if (steps <= h->coef_cap) return 0;
@@ -1571,14 +1596,26 @@ static int do_it_iters(gpe_handle* h, long n) {
     return finish_timing(h, ne);
 }
 
+// S1 / Tk buffers of real-time step parity par: the split plane pass reads
+// pair par and writes pair 1 - par (the unsplit pass works in place on s1 / tk).
+static void* rt_s1(gpe_handle* h, int par) { return (h->plane_split && par) ? h->s1b : h->s1; }
+static void* rt_tk(gpe_handle* h, int par) { return (h->plane_split && par) ? h->tkb : h->tk; }
+
 // Steady-state RT plane pass for step t -> t+1 (coupling from the table).
 template <typename T>
-static PlaneArgs<T> rt_steady_args(gpe_handle* h, int cidx, const int* cbase) {
+static PlaneArgs<T> rt_steady_args(gpe_handle* h, int cidx, const int* cbase, int par) {
     PlaneArgs<T> a = plane_args<T>(h, M_PINV | M_IN_TK | M_PHASE_FULL | M_RHO_FWD | M_KIN);
     a.cv = 2.0;
     a.coef = h->coef;
     a.cbase = cbase;
     a.cidx = cidx;
+    if (h->plane_split) {
+        a.split = 1;
+        a.s1 = (cplx<T>*)rt_s1(h, par);
+        a.tk = (cplx<T>*)rt_tk(h, par);
+        a.s1o = (cplx<T>*)rt_s1(h, 1 - par);
+        a.tko = (cplx<T>*)rt_tk(h, 1 - par);
+    }
     return a;
 }
 
@@ -1596,6 +1633,11 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
         cf[2 * t + 1] = (gm ? gm[t] : h->gamma) * p.w_15 / pow(dV, 1.5);
     }
     GPE_CHECK(hipMemcpyAsync(h->coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->plane_split && !h->s1b) {
+        const size_t cs = 2 * sizeof(T);
+        GPE_CHECK(hipMalloc(&h->s1b, (size_t)2 * p.ny * p.nx * (p.nz + 1) * cs));
+        GPE_CHECK(hipMalloc(&h->tkb, h->N * cs));
+    }
     // prologue: first half phase with the current fi3d, Poisson start, z/y kinetic
     PlaneArgs<T> a = plane_args<T>(h, M_PHI_LOAD | M_PHASE_HALF | M_RHO_FWD | M_KIN);
     a.in = (const cplx<T>*)h->store[h->cur];
@@ -1608,10 +1650,12 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
     const long steady = n - 1;
     if (h->use_graphs && !h->timing && steady >= kGraphSteps) {
         if (!h->rt_graph) {
+            // kGraphSteps is even: a replay ends on the buffer parity it started on
             const int rc = capture(h, &h->rt_graph, [&]() -> int {
                 for (int s = 0; s < kGraphSteps; ++s) {
-                    if (launch_line<T>(h, true, true)) return 1;
-                    if (launch_plane<T>(h, rt_steady_args<T>(h, s, h->cbase))) return 1;
+                    const int par = s & 1;
+                    if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
+                    if (launch_plane<T>(h, rt_steady_args<T>(h, s, h->cbase, par))) return 1;
                 }
                 hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->cbase, kGraphSteps);
                 GPE_CHECK(hipGetLastError());
@@ -1626,15 +1670,19 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
         }
     }
     for (; t < steady; ++t) {
-        if (launch_line<T>(h, true, true)) return 1;
+        const int par = (int)(t & 1);
+        if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
         ev_record(h, ne++);
-        if (launch_plane<T>(h, rt_steady_args<T>(h, (int)t, nullptr))) return 1;
+        if (launch_plane<T>(h, rt_steady_args<T>(h, (int)t, nullptr, par))) return 1;
         ev_record(h, ne++);
     }
     // last step: finish with a half phase and write psi, fi3d
-    if (launch_line<T>(h, true, true)) return 1;
+    const int par = (int)(steady & 1);
+    if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
     ev_record(h, ne++);
     a = plane_args<T>(h, M_PINV | M_PHI_STORE | M_IN_TK | M_PHASE_HALF | M_OUT);
+    a.s1 = a.s1o = (cplx<T>*)rt_s1(h, par);
+    a.tk = a.tko = (cplx<T>*)rt_tk(h, par);
     a.out = (cplx<T>*)h->store[h->cur];
     a.coef = h->coef;
     a.cidx = (int)(n - 1);
@@ -1925,6 +1973,8 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
         h->use_graphs = !(ng && ng[0] == '1');
         const char* ll = getenv("GPE_LINE_LDS");
         h->line_reg = !(ll && ll[0] == '1');
+        const char* ns = getenv("GPE_NO_SPLIT");
+        h->plane_split = !(ns && ns[0] == '1');
     }
     auto bad = [&](const std::string& m) {
         g_create_err = m;
@@ -1963,8 +2013,9 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
 void gpe_destroy(gpe_handle* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* bufs[] = {h->store[0], h->store[1], h->psii, h->V,  h->phi, h->s1,      h->tk,      h->tw,   h->ex,
-                    h->ey,       h->ez,       h->kx2,  h->ky2, h->kz2, h->part[0], h->part[1], h->epart};
+    void* bufs[] = {h->store[0], h->store[1], h->psii,     h->V,       h->phi,    h->s1,  h->tk,
+                    h->s1b,      h->tkb,      h->tw,       h->ex,      h->ey,     h->ez,  h->kx2,
+                    h->ky2,      h->kz2,      h->part[0],  h->part[1], h->epart};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);

@@ -63,3 +63,26 @@ def test_batch_graph_equals_direct(monkeypatch):
     assert np.array_equal(res["0"][1], res["1"][1])
     for u, v in zip(res["0"][2], res["1"][2]):
         assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("prec", [hip.GPE_FP64, hip.GPE_FP32])
+@pytest.mark.parametrize("shape", [(32, 24, 12), (64, 64, 16)])
+def test_split_plane_pass_equals_single(monkeypatch, prec, shape):
+    """Two blocks per plane with ping-pong S1 / Tk (the default real-time
+    plane pass) is bitwise the one-block-per-plane pass (GPE_NO_SPLIT=1),
+    for step counts of both parities, with and without graph replay."""
+    p, V, psi = case(shape)
+    outs = {}
+    for split in ("0", "1"):
+        monkeypatch.setenv("GPE_NO_SPLIT", split)
+        d = dev_for(p, V, psi, prec)
+        d.calc_fi3d()
+        r = []
+        for n in (1, 2, 67, 130):
+            d.rt_steps(n)
+            r.append(d.export_psi())
+        r.append(d.export_fi3d())
+        outs[split] = r
+        d.close()
+    for u, v in zip(outs["0"], outs["1"]):
+        assert np.array_equal(u, v)
