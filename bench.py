@@ -9,9 +9,13 @@ MULTIPLE_GAUSS(2) initial state with SplitMix64 phase noise (1e-3 rad, seed
 field, potential and all tables are resident in HBM before the timed region.
 
 Prints ONE JSON line.  Besides steps/s it reports:
-  roofline      the dominant kernel's achieved bandwidth (its own bytes per
-                launch / its average launch time measured with HIP events on
-                the solver's stream) against the 8 TB/s HBM peak;
+  roofline      the dominant kernel's achieved bandwidth (its algorithmic
+                bytes per launch / its average launch time, from start / stop
+                HIP events attached to each steady-state dispatch on the
+                solver's stream) against the 8 TB/s HBM peak; `traffic` is the
+                HBM bytes per launch that the FETCH_SIZE / WRITE_SIZE counter
+                runs recorded for that kernel (profiles/traffic_*.json, each
+                counter scaled by its reading for a copy of known size);
   step_roofline the model bytes of one RT step (112N + 20H, FP32) x steps/s
                 against the same peak;
   cpu_baseline  the FP64 CPU oracle (the project's restatement of
@@ -33,6 +37,17 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 GRID = (128, 128, 32)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rt_128x128x32_fp32.json")
+
+
+def measured_traffic(kernel: str):
+    """Counter-measured HBM bytes per launch of `kernel` (None if absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def build_case(precision):
@@ -80,10 +95,13 @@ def model_bytes(nx, ny, nz, fp64=False):
 
 
 def kernel_bytes(nx, ny, nz, cs=8, rs=4):
-    """Bytes each steady-state launch must move (its own inputs + outputs):
-    plane pass: read S1 + Tk + V, write S1 + Tk;  line pass: read+write S1 + Tk."""
+    """Algorithmic bytes of one steady-state real-time launch (its own inputs
+    and outputs, each once).  S1 holds 2ny x nx x nz complex values (kz =
+    1..nz), Tk N.  Plane pass, per x-plane: read S1 + Tk + V, write S1 + Tk
+    = 52 ny nz bytes in FP32, so 52 N per launch; line pass: read and write
+    S1 + Tk = 48 N per launch."""
     N = nx * ny * nz
-    s1 = cs * 2 * ny * nx * (nz + 1)
+    s1 = cs * 2 * ny * nx * nz
     tk = cs * N
     return {"plane": 2 * s1 + 2 * tk + rs * N, "line": 2 * s1 + 2 * tk}
 
@@ -135,6 +153,7 @@ def main():
 
     nx, ny, nz = GRID
     p, dev, V, psi = build_case(hip.GPE_FP32)
+    tuned = dev.autotune(200)  # launch-shape variant for this grid (scratch instance)
     # warm-up
     dev.rt_steps(args.warmup)
     dev.synchronize()
@@ -202,6 +221,7 @@ def main():
             "n_atoms": 2e4,
             "edd": 1.5,
             "parallelism": "single GPU",
+            "tuned": {k: tuned[k] for k in ("split_plane_pass", "register_x_pass")},
         },
         "roofline": {
             "bound": "hbm",
@@ -210,7 +230,7 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": measured_traffic(dom),
             "bytes_per_launch": kb[dom],
             "avg_launch_us": kern[dom],
             "other_kernel_us": {k: v for k, v in kern.items() if k != dom},

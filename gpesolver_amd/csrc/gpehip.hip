@@ -5,15 +5,20 @@
 //   psii      complex T [nx][ny][nz]   the reference's cpsii (its boundary is
 //                                      what every IT step writes there)
 //   V, fi3d   T [nx][ny][nz]           potential and dipolar potential (a.u.)
-//   S1        complex T [2ny][nx][nz+1]  Poisson spectrum after the z (R2C)
-//                                      and y passes, x still in real space
+//   S1        complex T [2ny][nx][nz]  Poisson spectrum after the z (R2C) and
+//                                      y passes, x still in real space, for
+//                                      kz = 1..nz: the kz = 0 column of
+//                                      rho^ * kz^2/k^2 is identically zero, so
+//                                      it is neither stored nor transformed
+//                                      (and rows are 2^a-aligned)
 //   Tk        complex T [ny][nx][nz]   kinetic split-step work array
+//   (the real-time plane pass ping-pongs S1 / Tk with a second pair)
 // The field is stored scaled, phi = sqrt(dV) psi, so that sum |phi|^2 = the
 // reference's xnorma and no FP32 value becomes subnormal (|psi|^5 ~ 1e-42 in
 // a.u. would); the nonlinear couplings absorb the powers of dV.
 //
 // Kernels (T = float or double):
-//   k_plane  one block per x-plane i: finishes the dipolar Poisson solve of the
+//   k_plane  one block (RT steady state: two) per x-plane i: finishes the dipolar Poisson solve of the
 //            plane (inverse y FFT, z C2R), applies the IT stencil or the RT
 //            potential phase, starts the next Poisson solve (z R2C, y FFT of
 //            N|psi|^2) and runs the kinetic propagation along z and y.
@@ -23,13 +28,17 @@
 // propagator exp(-i dt k^2/2m) = prod_ax exp(-i dt k_ax^2/2m) is separable, so
 // the 3D split step is three 1D (FFT, phase, inverse FFT) passes; the Poisson
 // multiplier kz^2/k^2 is not separable and keeps the 3-stage pipeline.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gpehip.h"
@@ -133,7 +142,7 @@ struct LineArgs {
     int ntile_p;  // Poisson tiles (first blocks)
     int cp;       // kz per Poisson tile
     int ck;       // k per kinetic tile
-    int tiles_kz;  // ceil((nz+1)/cp)
+    int tiles_kz;  // ceil(nz/cp): S1 columns kz = 1..nz
     int tiles_k;   // ceil(nz/ck)
     cplx<T>* s1;
     cplx<T>* tk;
@@ -240,7 +249,7 @@ struct PlanePolCT {
     template <typename T, int NT>
     static constexpr int rq() { return (NY * (NZ + 1) + NT - 1) / NT; }
     template <int NT>
-    static constexpr int rq2() { return (2 * NY * (NZ + 1) + NT - 1) / NT; }
+    static constexpr int rq2() { return (2 * NY * NZ + NT - 1) / NT; }
     template <typename T, int NT, int S>
     __device__ static void fft_y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
         gpe::ct::fft<T, 2 * NY, NT, NZ + 1, S>(b, nl, es, ls, tw + a.py2.tw);
@@ -326,10 +335,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
 #pragma unroll
             for (int q = 0; q < R2; ++q) {
                 const int e = threadIdx.x + q * NT;
-                if (e < ny2 * nzh) {
-                    const int ky = e / nzh, kz = e - ky * nzh;
-                    st[q] = a.s1[((size_t)ky * nx + i) * nzh + kz];
-                }
+                if (e < ny2 * nz) st[q] = a.s1[((size_t)(e / nz) * nx + i) * nz + e % nz];
             }
         }
 #pragma unroll
@@ -370,48 +376,95 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
 
     // ---- 1. Phi for this plane -------------------------------------------
     if (mode & M_PINV) {
-        // load S1[ky][i][kz] -> P[ky][kz]
+        // load S1[ky][i][kz-1] -> P[ky][kz], kz = 1..nz; the kz = 0 column
+        // of rho^ Vdd is identically zero (Vdd = kz^2/k^2) and is not stored
         if constexpr (kStage) {
 #pragma unroll
             for (int q = 0; q < R2; ++q) {
                 const int e = threadIdx.x + q * NT;
-                if (e < ny2 * nzh) P[e] = st[q];
+                if (e < ny2 * nz) P[(e / nz) * nzh + e % nz + 1] = st[q];
             }
         } else {
-            for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
-                const int ky = e / nzh, kz = e - ky * nzh;
-                P[e] = a.s1[((size_t)ky * nx + i) * nzh + kz];
+            for (int e = threadIdx.x; e < ny2 * nz; e += NT) {
+                const int ky = e / nz, c = e - ky * nz;
+                P[ky * nzh + c + 1] = a.s1[((size_t)ky * nx + i) * nz + c];
             }
         }
+        for (int ky = threadIdx.x; ky < ny2; ky += NT) P[ky * nzh] = gpe::mk(T(0), T(0));
         __syncthreads();
-        Pol::template fft_y2<T, NT, +1>(P, nzh, nzh, 1, a, tw);
-        // rows j < ny -> Z[j][0..2nz) with Hermitian extension, via registers
-        cplx<T> tmp[RQ];
-#pragma unroll
-        for (int q = 0; q < RQ; ++q) {
-            const int e = threadIdx.x + q * NT;
-            if (e < ny * nzh) tmp[q] = P[e];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < RQ; ++q) {
-            const int e = threadIdx.x + q * NT;
-            if (e < ny * nzh) {
-                const int j = e / nzh, kz = e - j * nzh;
-                P[j * zs + kz] = tmp[q];
-                if (kz >= 1 && kz < nz) P[j * zs + (nz2 - kz)] = gpe::mk(tmp[q].x, -tmp[q].y);
-            }
-        }
-        __syncthreads();
-        Pol::template fft_z2<T, NT, +1>(P, ny, 1, zs, a, tw);
+        Pol::template fft_y2<T, NT, +1>(P + 1, nz, nzh, 1, a, tw);
         const T sc = (T)(a.pinv_scale * s_in * s_in);
+        if constexpr (kStage) {
+            // z C2R, two rows per complex transform (ny is even for the
+            // compiled shapes): row m carries X_2m + i X_2m+1, each
+            // Hermitian-extended with real DC and Nyquist terms, so the real
+            // and imaginary parts of its inverse are the two real rows
+            constexpr int RQH = (RQ + 1) / 2;
+            const int nw = (ny / 2) * nzh;
+            cplx<T> xa[RQH], xb[RQH];
 #pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-            const int p = threadIdx.x + q * NT;
-            if (p < npl) {
-                const int j = p / nz, k = p - j * nz;
-                phi_r[q] = P[j * zs + k].x * sc;
-                if (mode & M_PHI_STORE) a.phi[plane_off + p] = phi_r[q];
+            for (int q = 0; q < RQH; ++q) {
+                const int w = threadIdx.x + q * NT;
+                if (w < nw) {
+                    const int m = w / nzh, kz = w - m * nzh;
+                    xa[q] = P[(2 * m) * nzh + kz];
+                    xb[q] = P[(2 * m + 1) * nzh + kz];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RQH; ++q) {
+                const int w = threadIdx.x + q * NT;
+                if (w < nw) {
+                    const int m = w / nzh, kz = w - m * nzh;
+                    if (kz == 0 || kz == nz) {
+                        P[m * zs + kz] = gpe::mk(xa[q].x, xb[q].x);
+                    } else {
+                        P[m * zs + kz] = gpe::mk(xa[q].x - xb[q].y, xa[q].y + xb[q].x);
+                        P[m * zs + (nz2 - kz)] = gpe::mk(xa[q].x + xb[q].y, xb[q].x - xa[q].y);
+                    }
+                }
+            }
+            __syncthreads();
+            Pol::template fft_z2<T, NT, +1>(P, ny / 2, 1, zs, a, tw);
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < npl) {
+                    const int j = p / nz, k = p - j * nz;
+                    const cplx<T> v = P[(j >> 1) * zs + k];
+                    phi_r[q] = ((j & 1) ? v.y : v.x) * sc;
+                    if (mode & M_PHI_STORE) a.phi[plane_off + p] = phi_r[q];
+                }
+            }
+        } else {
+            // rows j < ny -> Z[j][0..2nz) with Hermitian extension, via registers
+            cplx<T> tmp[RQ];
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int e = threadIdx.x + q * NT;
+                if (e < ny * nzh) tmp[q] = P[e];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int e = threadIdx.x + q * NT;
+                if (e < ny * nzh) {
+                    const int j = e / nzh, kz = e - j * nzh;
+                    P[j * zs + kz] = tmp[q];
+                    if (kz >= 1 && kz < nz) P[j * zs + (nz2 - kz)] = gpe::mk(tmp[q].x, -tmp[q].y);
+                }
+            }
+            __syncthreads();
+            Pol::template fft_z2<T, NT, +1>(P, ny, 1, zs, a, tw);
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < npl) {
+                    const int j = p / nz, k = p - j * nz;
+                    phi_r[q] = P[j * zs + k].x * sc;
+                    if (mode & M_PHI_STORE) a.phi[plane_off + p] = phi_r[q];
+                }
             }
         }
         __syncthreads();
@@ -580,25 +633,55 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
 
     // ---- 3. start the Poisson solve of rho = |psi|^2 ----------------------
     if (mode & M_RHO_FWD) {
-        // Z[j][k] = rho for k < nz, 0 for k in [nz, 2nz)
-#pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-            const int p = threadIdx.x + q * NT;
-            if (p < npl) {
-                const int j = p / nz, k = p - j * nz;
-                P[j * zs + k] = gpe::mk(rho_r[q], T(0));
-                P[j * zs + nz + k] = gpe::mk(T(0), T(0));
-            }
-        }
-        __syncthreads();
-        Pol::template fft_z2<T, NT, -1>(P, ny, 1, zs, a, tw);
         cplx<T> tmp[RQ];
+        if constexpr (kStage) {
+            // z R2C, two rows per complex transform: row m = rho_2m + i rho_2m+1
+            // (zero for k in [nz, 2nz)); the spectra separate as
+            // A = (Z_k + conj Z_-k) / 2, B = (Z_k - conj Z_-k) / 2i
 #pragma unroll
-        for (int q = 0; q < RQ; ++q) {
-            const int e = threadIdx.x + q * NT;
-            if (e < ny * nzh) {
-                const int j = e / nzh, kz = e - j * nzh;
-                tmp[q] = P[j * zs + kz];
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < npl) {
+                    const int j = p / nz, k = p - j * nz;
+                    reinterpret_cast<T*>(&P[(j >> 1) * zs + k])[j & 1] = rho_r[q];
+                    if (!(j & 1)) P[(j >> 1) * zs + nz + k] = gpe::mk(T(0), T(0));
+                }
+            }
+            __syncthreads();
+            Pol::template fft_z2<T, NT, -1>(P, ny / 2, 1, zs, a, tw);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int e = threadIdx.x + q * NT;
+                if (e < ny * nzh) {
+                    const int j = e / nzh, kz = e - j * nzh;
+                    const cplx<T> zk = P[(j >> 1) * zs + kz];
+                    const cplx<T> zm = P[(j >> 1) * zs + (kz == 0 ? 0 : nz2 - kz)];
+                    if (j & 1)
+                        tmp[q] = gpe::mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
+                    else
+                        tmp[q] = gpe::mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
+                }
+            }
+        } else {
+            // Z[j][k] = rho for k < nz, 0 for k in [nz, 2nz)
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < npl) {
+                    const int j = p / nz, k = p - j * nz;
+                    P[j * zs + k] = gpe::mk(rho_r[q], T(0));
+                    P[j * zs + nz + k] = gpe::mk(T(0), T(0));
+                }
+            }
+            __syncthreads();
+            Pol::template fft_z2<T, NT, -1>(P, ny, 1, zs, a, tw);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int e = threadIdx.x + q * NT;
+                if (e < ny * nzh) {
+                    const int j = e / nzh, kz = e - j * nzh;
+                    tmp[q] = P[j * zs + kz];
+                }
             }
         }
         __syncthreads();
@@ -609,10 +692,10 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         }
         for (int e = ny * nzh + threadIdx.x; e < ny2 * nzh; e += NT) P[e] = gpe::mk(T(0), T(0));
         __syncthreads();
-        Pol::template fft_y2<T, NT, -1>(P, nzh, nzh, 1, a, tw);
-        for (int e = threadIdx.x; e < ny2 * nzh; e += NT) {
-            const int ky = e / nzh, kz = e - ky * nzh;
-            a.s1o[((size_t)ky * nx + i) * nzh + kz] = P[e];
+        Pol::template fft_y2<T, NT, -1>(P + 1, nz, nzh, 1, a, tw);
+        for (int e = threadIdx.x; e < ny2 * nz; e += NT) {
+            const int ky = e / nz, c = e - ky * nz;
+            a.s1o[((size_t)ky * nx + i) * nz + c] = P[ky * nzh + c + 1];
         }
     }
 }
@@ -622,17 +705,17 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
 template <typename T, int NT, class Pol>
 __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned char* lds_raw) {
     cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
-    const int nx = Pol::nx(a), nzh = a.nz + 1;
+    const int nx = Pol::nx(a), nzs = a.nz;  // S1 rows hold kz = 1..nz
     if (b < a.ntile_p) {
         const int ky = b / a.tiles_kz;
-        const int kz0 = (b - ky * a.tiles_kz) * a.cp;
-        const int C = min(a.cp, nzh - kz0);
+        const int kz0 = (b - ky * a.tiles_kz) * a.cp;  // column index, kz = kz0 + c + 1
+        const int C = min(a.cp, nzs - kz0);
         const int nx2 = 2 * nx;
         const int CS = C + 1;  // padded LDS stride between x points
-        cplx<T>* base = a.s1 + (size_t)ky * nx * nzh + kz0;
+        cplx<T>* base = a.s1 + (size_t)ky * nx * nzs + kz0;
         for (int e = threadIdx.x; e < nx2 * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            X[ii * CS + c] = (ii < nx) ? base[(size_t)ii * nzh + c] : gpe::mk(T(0), T(0));
+            X[ii * CS + c] = (ii < nx) ? base[(size_t)ii * nzs + c] : gpe::mk(T(0), T(0));
         }
         __syncthreads();
         Pol::template fft_x2<T, NT, -1>(X, C, CS, 1, a);
@@ -640,7 +723,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
         const T ky2 = (T)a.ky2[ky];
         for (int e = threadIdx.x; e < nx2 * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            const T kz2 = (T)a.kz2[kz0 + c];
+            const T kz2 = (T)a.kz2[kz0 + c + 1];
             const T k2 = (T)a.kx2[ii] + ky2 + kz2;
             const T v = (k2 > T(1e-30)) ? kz2 / k2 : T(0);
             X[ii * CS + c] = gpe::cscale(X[ii * CS + c], v);
@@ -649,7 +732,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
         Pol::template fft_x2<T, NT, +1>(X, C, CS, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            base[(size_t)ii * nzh + c] = X[ii * CS + c];
+            base[(size_t)ii * nzs + c] = X[ii * CS + c];
         }
     } else {
         b -= a.ntile_p;
@@ -721,18 +804,19 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
     constexpr int A = 16, B = L / A, R = A / B;
     constexpr int LS = LPB + 1;  // padded row stride of the exchange area
     static_assert(L % A == 0 && A % B == 0, "line split");
-    const int nx = a.nx, nzh = a.nz + 1, nz = a.nz;
-    const int nlines = POISSON ? 2 * a.ny * nzh : a.ny * nz;
+    const int nx = a.nx, nz = a.nz;
+    const int nlines = POISSON ? 2 * a.ny * nz : a.ny * nz;
     const int ll = threadIdx.x % LPB, b = threadIdx.x / LPB;
     const int line = blk * LPB + ll;
     const bool valid = line < nlines;
     cplx<T>* base;
     int estride, ky = 0, kz = 0;
     if (POISSON) {
-        ky = valid ? line / nzh : 0;
-        kz = valid ? line - ky * nzh : 0;
-        base = a.s1 + (size_t)ky * nx * nzh + kz;
-        estride = nzh;
+        ky = valid ? line / nz : 0;
+        const int c = valid ? line - ky * nz : 0;  // S1 rows hold kz = 1..nz
+        kz = c + 1;
+        base = a.s1 + (size_t)ky * nx * nz + c;
+        estride = nz;
     } else {
         const int j = valid ? line / nz : 0;
         const int k = valid ? line - j * nz : 0;
@@ -1093,6 +1177,12 @@ struct gpe_handle {
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
+    // Per-launch start / stop events of the steady-state plane (0) and line
+    // (1) launches, attached to the dispatches themselves
+    // (hipExtLaunchKernelGGL), so they time exactly the kernel.
+    bool tim_on = false;
+    std::vector<hipEvent_t> kev[2][2];
+    size_t kn[2] = {0, 0};
     double t_plane_us = 0, t_line_us = 0, t_total_ms = 0;
 };
 
@@ -1279,7 +1369,7 @@ static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
     a.generic = h->force_generic ? 1 : 0;
     a.cp = h->cp;
     a.ck = h->ck;
-    a.tiles_kz = (p.nz + 1 + h->cp - 1) / h->cp;
+    a.tiles_kz = (p.nz + h->cp - 1) / h->cp;  // S1 columns kz = 1..nz
     a.tiles_k = (p.nz + h->ck - 1) / h->ck;
     a.ntile_p = poisson ? 2 * p.ny * a.tiles_kz : 0;
     a.s1 = (cplx<T>*)h->s1;
@@ -1293,6 +1383,22 @@ static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
     a.px = h->px;
     (void)kinetic;
     return a;
+}
+
+// Start / stop events for the next timed launch of kind k (0 plane, 1 line),
+// or nulls when this launch is not timed.
+static void kernel_events(gpe_handle* h, int k, hipEvent_t* e0, hipEvent_t* e1) {
+    *e0 = *e1 = nullptr;
+    if (!h->timing || !h->tim_on) return;
+    const size_t i = h->kn[k]++;
+    for (int w = 0; w < 2; ++w)
+        while (h->kev[k][w].size() <= i) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            h->kev[k][w].push_back(e);
+        }
+    *e0 = h->kev[k][0][i];
+    *e1 = h->kev[k][1][i];
 }
 
 // The plane shape has a compiled kernel (PlanePolCT) that the launches use.
@@ -1310,7 +1416,10 @@ template <typename T, class Pol>
 static int launch_plane_pol(gpe_handle* h, const PlaneArgs<T>& a) {
     constexpr int NT = PlaneThreads<T, Pol::kStage>::value;
     const size_t lds = h->plane_lds + (Pol::kStage ? h->plane_stage : 0);
-    hipLaunchKernelGGL((k_plane<T, NT, Pol>), dim3(h->p.nx * (a.split ? 2 : 1)), dim3(NT), lds, h->stream, a);
+    hipEvent_t e0, e1;
+    kernel_events(h, 0, &e0, &e1);
+    hipExtLaunchKernelGGL((k_plane<T, NT, Pol>), dim3(h->p.nx * (a.split ? 2 : 1)), dim3(NT), lds, h->stream, e0, e1,
+                          0, a);
     GPE_CHECK(hipGetLastError());
     return 0;
 }
@@ -1330,7 +1439,10 @@ static int launch_plane(gpe_handle* h, const PlaneArgs<T>& a) {
 
 template <typename T, class Pol>
 static int launch_line_pol(gpe_handle* h, const LineArgs<T>& a, int nblk) {
-    hipLaunchKernelGGL((k_xpass<T, kLineThreads, Pol>), dim3(nblk), dim3(kLineThreads), h->line_lds, h->stream, a);
+    hipEvent_t e0, e1;
+    kernel_events(h, 1, &e0, &e1);
+    hipExtLaunchKernelGGL((k_xpass<T, kLineThreads, Pol>), dim3(nblk), dim3(kLineThreads), h->line_lds, h->stream, e0,
+                          e1, 0, a);
     GPE_CHECK(hipGetLastError());
     return 0;
 }
@@ -1338,11 +1450,13 @@ static int launch_line_pol(gpe_handle* h, const LineArgs<T>& a, int nblk) {
 template <typename T, int NX>
 static int launch_line_reg(gpe_handle* h, const LineArgs<T>& a, bool poisson, bool kinetic) {
     using X = XReg<NX>;
-    const int np = poisson ? (2 * h->p.ny * (h->p.nz + 1) + X::LPB_P - 1) / X::LPB_P : 0;
+    const int np = poisson ? (2 * h->p.ny * h->p.nz + X::LPB_P - 1) / X::LPB_P : 0;
     const int nk = kinetic ? (h->p.ny * h->p.nz + X::LPB_K - 1) / X::LPB_K : 0;
     if (np + nk == 0) return 0;
     const size_t lds = (size_t)(16 * 256 + 256 + 4 * NX) * sizeof(cplx<T>);
-    hipLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, a, np);
+    hipEvent_t e0, e1;
+    kernel_events(h, 1, &e0, &e1);
+    hipExtLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, e0, e1, 0, a, np);
     GPE_CHECK(hipGetLastError());
     return 0;
 }
@@ -1465,26 +1579,22 @@ static void ev_record(gpe_handle* h, size_t idx) {
 }
 
 static int finish_timing(gpe_handle* h, size_t nev) {
+    h->tim_on = false;
     if (!h->timing || nev < 2) return 0;
     GPE_CHECK(hipEventSynchronize(h->ev[nev - 1]));
-    // events: [start, (after line, after plane)*]
-    double tl = 0, tp = 0;
-    int nl = 0, np = 0;
-    for (size_t t = 1; t < nev; ++t) {
-        float ms = 0;
-        GPE_CHECK(hipEventElapsedTime(&ms, h->ev[t - 1], h->ev[t]));
-        if (t % 2 == 1) {
-            tl += ms;
-            ++nl;
-        } else {
-            tp += ms;
-            ++np;
+    double t[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+        for (size_t i = 0; i < h->kn[k]; ++i) {
+            float ms = 0;
+            GPE_CHECK(hipEventElapsedTime(&ms, h->kev[k][0][i], h->kev[k][1][i]));
+            t[k] += ms;
         }
+        t[k] = h->kn[k] ? t[k] * 1e3 / h->kn[k] : 0;
     }
     float tot = 0;
     GPE_CHECK(hipEventElapsedTime(&tot, h->ev[0], h->ev[nev - 1]));
-    h->t_line_us = nl ? tl * 1e3 / nl : 0;
-    h->t_plane_us = np ? tp * 1e3 / np : 0;
+    h->t_plane_us = t[0];
+    h->t_line_us = t[1];
     h->t_total_ms = tot;
     return 0;
 }
@@ -1577,6 +1687,8 @@ static int do_it_iters(gpe_handle* h, long n) {
             steady -= kGraphSteps;  // an even number of steps: cur / pcur unchanged
         }
     }
+    h->kn[0] = h->kn[1] = 0;
+    h->tim_on = true;  // time the steady-state launches
     for (long t = 0; t < steady; ++t) {
         if (launch_line<T>(h, true, false)) return 1;
         ev_record(h, ne++);
@@ -1591,6 +1703,7 @@ static int do_it_iters(gpe_handle* h, long n) {
         h->cur = 1 - h->cur;
         h->pcur = 1 - h->pcur;
     }
+    h->tim_on = false;
     if (it_step_launch<T>(h, true)) return 1;
     h->norm_pending = true;
     return finish_timing(h, ne);
@@ -1635,7 +1748,7 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
     GPE_CHECK(hipMemcpyAsync(h->coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (h->plane_split && !h->s1b) {
         const size_t cs = 2 * sizeof(T);
-        GPE_CHECK(hipMalloc(&h->s1b, (size_t)2 * p.ny * p.nx * (p.nz + 1) * cs));
+        GPE_CHECK(hipMalloc(&h->s1b, (size_t)2 * p.ny * p.nx * p.nz * cs));
         GPE_CHECK(hipMalloc(&h->tkb, h->N * cs));
     }
     // prologue: first half phase with the current fi3d, Poisson start, z/y kinetic
@@ -1669,6 +1782,8 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
             t += kGraphSteps;
         }
     }
+    h->kn[0] = h->kn[1] = 0;
+    h->tim_on = true;  // time the steady-state launches
     for (; t < steady; ++t) {
         const int par = (int)(t & 1);
         if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
@@ -1676,6 +1791,7 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
         if (launch_plane<T>(h, rt_steady_args<T>(h, (int)t, nullptr, par))) return 1;
         ev_record(h, ne++);
     }
+    h->tim_on = false;
     // last step: finish with a half phase and write psi, fi3d
     const int par = (int)(steady & 1);
     if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
@@ -1984,7 +2100,7 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bad("stream create failed");
     const size_t cs = csize(h), rs = rsize(h);
     const size_t n = h->N;
-    const size_t ns1 = (size_t)2 * p->ny * p->nx * (p->nz + 1);
+    const size_t ns1 = (size_t)2 * p->ny * p->nx * p->nz;  // kz = 1..nz (the kz = 0 column is zero)
     if (hipMalloc(&h->store[0], n * cs) || hipMalloc(&h->store[1], n * cs) || hipMalloc(&h->psii, n * cs) ||
         hipMalloc(&h->V, n * rs) || hipMalloc(&h->phi, n * rs) || hipMalloc(&h->s1, ns1 * cs) ||
         hipMalloc(&h->tk, n * cs) || hipMalloc(&h->part[0], p->nx * 8) || hipMalloc(&h->part[1], p->nx * 8) ||
@@ -1998,7 +2114,7 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     if (rc) return bad(h->err);
     // launch shapes: Poisson tiles of cp kz-columns, kinetic tiles of ck k
     if (2L * p->nx > kLineMaxE) return bad("x length too large for the line pass");
-    h->cp = std::min(p->nz + 1, std::max(1, kLineMaxE / (2 * p->nx)));
+    h->cp = std::min(p->nz, std::max(1, kLineMaxE / (2 * p->nx)));
     h->cp = std::min(h->cp, 12);
     h->ck = std::min(p->nz, std::max(1, kLineMaxE / p->nx));
     h->ck = std::min(h->ck, 16);
@@ -2019,6 +2135,9 @@ void gpe_destroy(gpe_handle* h) {
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+    for (auto& kk : h->kev)
+        for (auto& v : kk)
+            for (hipEvent_t e : v) (void)hipEventDestroy(e);
     if (h->rt_graph) (void)hipGraphExecDestroy(h->rt_graph);
     for (hipGraphExec_t g : h->it_graph)
         if (g) (void)hipGraphExecDestroy(g);
@@ -2150,6 +2269,109 @@ int gpe_batch_it_until(gpe_batch* b, long max_steps, long check_every, double to
     b->pcur = 0;
     return b->prec == GPE_FP64 ? batch_it_until<double>(b, max_steps, check_every, tol, steps_out, energy_per_atom_out)
                                : batch_it_until<float>(b, max_steps, check_every, tol, steps_out, energy_per_atom_out);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner.  The launch-shape variants of the fused loops are timed on a
+// scratch instance of the same grid and precision (uniform field; the
+// kernels have no data-dependent control flow), and the fastest is kept for
+// h and cached per (grid, precision) for the process.  Variants: one or two
+// blocks per plane in the real-time plane pass (bitwise identical results),
+// and the register-resident or the LDS x pass (the same transform, rounded
+// differently).  Untuned handles use the defaults (split, register x pass).
+static void drop_graphs(gpe_handle* h) {
+    if (h->rt_graph) (void)hipGraphExecDestroy(h->rt_graph);
+    h->rt_graph = nullptr;
+    for (auto& g : h->it_graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+struct TuneResult {
+    bool split, line_reg;
+    double rt_us, it_us;
+};
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, TuneResult> g_tune;
+
+static int tune_time(gpe_handle* h, long steps, double* rt_us, double* it_us) {
+    gpe_handle* s = h;  // the scratch instance
+    auto run = [&](bool rt, long n) -> int {
+        return rt ? DISPATCH(s, do_rt_iters, s, n, nullptr, nullptr) : DISPATCH(s, do_it_iters, s, n);
+    };
+    for (int m = 0; m < 2; ++m) {
+        const bool rt = m == 0;
+        if (run(rt, kGraphSteps + 2)) return 1;  // warm-up, captures the graphs
+        GPE_CHECK(hipStreamSynchronize(s->stream));
+        hipEvent_t e0, e1;
+        GPE_CHECK(hipEventCreate(&e0));
+        GPE_CHECK(hipEventCreate(&e1));
+        (void)hipEventRecord(e0, s->stream);
+        const int rc = run(rt, steps);
+        (void)hipEventRecord(e1, s->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return 1;
+        (rt ? *rt_us : *it_us) = 1e3 * ms / steps;
+    }
+    return 0;
+}
+
+int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, double* it_step_us) {
+    if (!h) return 1;
+    if (steps < 1) return fail(h, "autotune needs steps >= 1");
+    const auto key = std::make_tuple(h->p.nx, h->p.ny, h->p.nz, h->prec);
+    TuneResult best{};
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            best = it->second;
+            have = true;
+        }
+    }
+    if (!have) {
+        gpe_handle* s = nullptr;
+        if (gpe_create(&h->p, h->prec, &s)) return fail(h, std::string("autotune scratch instance: ") + g_create_err);
+        std::vector<double> psi(2 * h->N, 1.0);
+        int rc = gpe_load_buffer(s, psi.data());
+        const bool reg_ok = !s->force_generic && (h->p.nx == 128 || h->p.nx == 64);
+        double best_cost = 0;
+        for (int sp = 1; sp >= 0 && !rc; --sp)
+            for (int lr = reg_ok ? 1 : 0; lr >= 0 && !rc; --lr) {
+                s->plane_split = sp != 0;
+                s->line_reg = lr != 0;
+                drop_graphs(s);
+                double rt = 0, itu = 0;
+                rc = tune_time(s, steps, &rt, &itu);
+                const double cost = rt + itu;
+                if (!rc && (!have || cost < best_cost)) {
+                    best = TuneResult{sp != 0, lr != 0, rt, itu};
+                    best_cost = cost;
+                    have = true;
+                }
+            }
+        if (rc) {
+            const std::string m = s->err;
+            gpe_destroy(s);
+            return fail(h, "autotune: " + m);
+        }
+        gpe_destroy(s);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = best;
+    }
+    if (h->plane_split != best.split || h->line_reg != best.line_reg) drop_graphs(h);
+    h->plane_split = best.split;
+    h->line_reg = best.line_reg;
+    if (config) *config = (best.split ? 1 : 0) | (best.line_reg ? 2 : 0);
+    if (rt_step_us) *rt_step_us = best.rt_us;
+    if (it_step_us) *it_step_us = best.it_us;
+    return 0;
 }
 
 int gpe_set_timing(gpe_handle* h, int on) {

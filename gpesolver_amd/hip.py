@@ -23,6 +23,7 @@ EXPORTS = (
     "gpe_set_coupling", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_synchronize", "gpe_set_timing", "gpe_get_timing",
+    "gpe_autotune",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
     "gpe_fno_create", "gpe_fno_destroy", "gpe_fno_last_error", "gpe_fno_num_params", "gpe_fno_get_params",
     "gpe_fno_set_params", "gpe_fno_get_grads", "gpe_fno_forward", "gpe_fno_loss_grad", "gpe_fno_adam_step",
@@ -90,6 +91,8 @@ def load(path: str = LIB_PATH):
     L.gpe_imag_time_iters.restype = ip
     L.gpe_real_time_iters.argtypes = [vp, C.c_long, dp, dp]
     L.gpe_real_time_iters.restype = ip
+    L.gpe_autotune.argtypes = [vp, C.c_long, C.POINTER(C.c_int), dp, dp]
+    L.gpe_autotune.restype = ip
     L.gpe_set_timing.argtypes = [vp, ip]
     L.gpe_set_timing.restype = ip
     L.gpe_get_timing.argtypes = [vp, dp, dp, dp]
@@ -220,6 +223,14 @@ class Device:
 
     def synchronize(self):
         self._check(self.L.gpe_synchronize(self.h))
+
+    def autotune(self, steps: int = 200) -> dict:
+        """Pick the fastest launch-shape variant for this grid (gpe_autotune)."""
+        cfg = C.c_int()
+        rt, it = C.c_double(), C.c_double()
+        self._check(self.L.gpe_autotune(self.h, steps, C.byref(cfg), C.byref(rt), C.byref(it)))
+        return {"split_plane_pass": bool(cfg.value & 1), "register_x_pass": bool(cfg.value & 2),
+                "rt_step_us": rt.value, "it_step_us": it.value}
 
     def set_timing(self, on: bool):
         self._check(self.L.gpe_set_timing(self.h, 1 if on else 0))
