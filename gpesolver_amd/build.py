@@ -12,6 +12,7 @@ EXE = os.path.join(HERE, "gpesolver")
 SOURCES = [os.path.join(HERE, "csrc", "gpehip.hip"), os.path.join(HERE, "csrc", "gpe_fno.hip")]
 HOST_SOURCES = [os.path.join(HERE, "host", "gpe_host.cpp"), os.path.join(HERE, "host", "gpe_host_abi.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "gpe_fft.hpp"), os.path.join(HERE, "csrc", "gpe_fft_ct.hpp"),
+           os.path.join(HERE, "csrc", "gpe_sincos.hpp"),
            os.path.join(HERE, "csrc", "gpe_gemm.hpp"),
            os.path.join(HERE, "host", "gpe_host.hpp"), os.path.join(ROOT, "include", "gpehip.h"),
            os.path.join(ROOT, "include", "gpesolver_host.h")]

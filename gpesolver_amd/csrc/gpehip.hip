@@ -44,6 +44,7 @@
 #include "../../include/gpehip.h"
 #include "gpe_fft.hpp"
 #include "gpe_fft_ct.hpp"
+#include "gpe_sincos.hpp"
 
 using gpe::cplx;
 using gpe::LinePlan;
@@ -154,15 +155,11 @@ struct LineArgs {
     LinePlan px2, px;
 };
 
+// rho^(3/2) for the LHY phase term, in FP64, from a density of precision T
+// (the square root in T: FP32 for the bulk path).
 template <typename T>
-__device__ __forceinline__ void sincos_t(T a, T* s, T* c);
-template <>
-__device__ __forceinline__ void sincos_t<float>(float a, float* s, float* c) {
-    sincosf(a, s, c);
-}
-template <>
-__device__ __forceinline__ void sincos_t<double>(double a, double* s, double* c) {
-    sincos(a, s, c);
+__device__ __forceinline__ double rho32(T nrm) {
+    return (double)nrm * (double)sqrt(nrm);
 }
 
 // Deterministic block sum of one double per thread (fixed tree).
@@ -208,6 +205,8 @@ __device__ double sum_partials(const double* p, int n, double* red) {
 // radix-4/2/3 engine) or as compile-time constants (large-radix engine).
 struct PlanePolGeneric {
     static constexpr bool kStage = false;  // tables stay in global memory
+    static constexpr int kNY = 0, kNZ = 0;
+    static constexpr bool kRegY = false, kRegZ = false, kRegY2 = false;
     template <class A>
     __device__ static int ny(const A& a) { return a.ny; }
     template <class A>
@@ -240,6 +239,13 @@ template <int NY, int NZ>
 struct PlanePolCT {
     // twiddle / kinetic tables and the psi, V loads are staged at block start
     static constexpr bool kStage = true;
+    static constexpr int kNY = NY, kNZ = NZ;
+    // register-resident kinetic y pass (four-step, NY = 16 B with B | 16)
+    static constexpr bool kRegY = (NY == 128 || NY == 64) && NY / 16 * NZ <= 512;
+    // register-resident kinetic z pass (one or two lanes per row)
+    static constexpr bool kRegZ = NZ <= 16 || NZ == 32 || NZ == 24;
+    // register-resident y transforms of the Poisson stages
+    static constexpr bool kRegY2 = (NY == 128 || NY == 64) && 2 * NY / 16 * NZ <= 512;
     template <class A>
     __device__ static constexpr int ny(const A&) { return NY; }
     template <class A>
@@ -296,6 +302,262 @@ struct LinePolCT {
 };
 
 // ---------------------------------------------------------------------------
+// Register-resident kinetic propagation inside the plane kernel (compiled
+// shapes).  Each 1-D line runs forward DFT -> x phase table -> inverse DFT
+// without LDS round trips between the stages.
+
+// Value of the neighbouring lane (lane ^ 1) through a DPP quad permutation
+// [1, 0, 3, 2]: a VALU operand modifier, no LDS round trip.
+__device__ __forceinline__ float dpp_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double dpp_xor1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> shfl_xor1(cplx<T> v) {
+    return gpe::mk(dpp_xor1(v.x), dpp_xor1(v.y));
+}
+
+// z: rows Q[j][0..NZ) (row stride qs), j < ny.  NZ <= 16: one thread per
+// row.  NZ = 2A (A = 16 or 12): two lanes per row, lane h holding
+// x[2a + h]; DFT_A over a, twiddle W_NZ^(h k1), and the radix-2 stage across
+// the lane pair through shuffles (thread h then holds X[k1 + A h]); the
+// inverse runs the same steps backwards.  twz = W_NZ^m, ez = phase / NZ.
+template <typename T, int NZ, int NT>
+__device__ __forceinline__ void kin_z_reg(cplx<T>* Q, int ny, int qs, const cplx<T>* ez, const cplx<T>* twz) {
+    if constexpr (NZ <= 16) {
+        for (int j = threadIdx.x; j < ny; j += NT) {
+            cplx<T> v[NZ];
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) v[k] = Q[j * qs + k];
+            gpe::ct::Dft<T, NZ, -1>::run(v);
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) v[k] = gpe::cmul(v[k], ez[k]);
+            gpe::ct::Dft<T, NZ, +1>::run(v);
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) Q[j * qs + k] = v[k];
+        }
+    } else {
+        constexpr int A = NZ / 2;
+        static_assert(NZ == 2 * A && (A == 16 || A == 12), "z length of the paired register pass");
+        const int nthr = 2 * ny;
+        for (int t0 = 0; t0 < nthr; t0 += NT) {
+            if (t0 + (int)(threadIdx.x & ~63u) >= nthr) continue;  // whole waves only: shuffles below
+            const int t = t0 + threadIdx.x;
+            const bool act = t < nthr;
+            const int j = act ? (t >> 1) : 0, h = t & 1;
+            cplx<T> v[A];
+#pragma unroll
+            for (int q = 0; q < A; ++q) v[q] = act ? Q[j * qs + 2 * q + h] : gpe::mk(T(0), T(0));
+            gpe::ct::Dft<T, A, -1>::run(v);
+            if (h) {
+#pragma unroll
+                for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmul(v[k1], twz[k1]);
+            }
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) {
+                const cplx<T> o = shfl_xor1(v[k1]);
+                v[k1] = h ? gpe::csub(o, v[k1]) : gpe::cadd(v[k1], o);
+                v[k1] = gpe::cmul(v[k1], ez[k1 + A * h]);
+            }
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) {
+                const cplx<T> o = shfl_xor1(v[k1]);
+                v[k1] = h ? gpe::csub(o, v[k1]) : gpe::cadd(v[k1], o);
+            }
+            if (h) {
+#pragma unroll
+                for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmulc(v[k1], twz[k1]);
+            }
+            gpe::ct::Dft<T, A, +1>::run(v);
+            if (act) {
+#pragma unroll
+                for (int q = 0; q < A; ++q) Q[j * qs + 2 * q + h] = v[q];
+            }
+        }
+    }
+}
+
+// y: columns c < NZ of Q[j][c] (row stride qs), length NY = 16 B (B = 4 or
+// 8), four-step with one LDS exchange each way: thread (b, c) holds rows
+// j = B a + b; DFT_16 over a, twiddle W_NY^(b k1), exchange, DFT_B over b for
+// its rows k1 = b + B r, x phase ey, inverse DFT_B, conjugate twiddle,
+// exchange back, inverse DFT_16; the result goes straight from registers
+// to Tk[j][i][c] (global).  The exchange area aliases Q.  Every thread of the
+// block reaches the barriers.
+template <typename T, int NY, int NZ, int NT>
+__device__ __forceinline__ void kin_y_reg_store(cplx<T>* Q, int qs, const cplx<T>* ey, const cplx<T>* twy,
+                                                cplx<T>* tk_plane, int nx) {
+    constexpr int A = 16, B = NY / A, R = A / B;
+    static_assert(NY == A * B && A % B == 0 && B * NZ <= NT, "y length of the register pass");
+    constexpr int ES = NZ + 1;  // exchange row stride (odd: distinct banks)
+    const bool act = threadIdx.x < B * NZ;
+    const int c = threadIdx.x % NZ, b = threadIdx.x / NZ;
+    cplx<T> v[A];
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < A; ++q) v[q] = Q[(B * q + b) * qs + c];
+        gpe::ct::Dft<T, A, -1>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmul(v[k1], twy[(b * k1) % NY]);
+    }
+    __syncthreads();  // Q is read: the exchange area may overwrite it
+    cplx<T>* E = Q;
+    if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * ES + c] = v[k1];
+    }
+    __syncthreads();
+    cplx<T> w[R][B];
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k1 = b + B * r;
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * ES + c];
+            gpe::ct::Dft<T, B, -1>::run(w[r]);
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::cmul(w[r][k2], ey[k1 + A * k2]);
+            gpe::ct::Dft<T, B, +1>::run(w[r]);
+#pragma unroll
+            for (int bb = 1; bb < B; ++bb) w[r][bb] = gpe::cmulc(w[r][bb], twy[(bb * k1) % NY]);
+        }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k1 = b + B * r;
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) E[(k1 * B + bb) * ES + c] = w[r][bb];
+        }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * ES + c];
+        gpe::ct::Dft<T, A, +1>::run(v);
+#pragma unroll
+        for (int q = 0; q < A; ++q) tk_plane[(size_t)(B * q + b) * nx * NZ + c] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The 2NY-point y transforms of the Poisson stages in registers (compiled
+// shapes with NY = 64 or 128): four-step with A = 16 values per thread and
+// B = 2NY/16 threads per kz column (kz = c + 1, c < NZ), one LDS exchange
+// (aliasing the plane tile).  Thread (b, c) holds y = B a + b on input and
+// rows k1 = b + B r (r < 16/B), outputs k1 + 16 k2, after the exchange.
+template <int NY, int NZ>
+struct Y2Reg {
+    static constexpr int L = 2 * NY, A = 16, B = L / A, R = A / B, ES = NZ + 1, NTH = B * NZ;
+    static_assert(L == A * B && A % B == 0 && NY == 8 * B, "y length of the register Poisson pass");
+};
+
+// Inverse y of S1 column c (v[a] = S1[B a + b][i][c], loaded by the caller),
+// then the z C2R input rows of the packed pairs (j, j + NY/2) straight from
+// registers: P[j][kz] = X_j + i X_j+NY/2 and its Hermitian mirror at 2NZ - kz
+// (real DC and Nyquist terms; the kz = 0 column is zero).  tw2 = W_2NY^m.
+// The caller places a barrier before the z transform.
+template <typename T, int NY, int NZ>
+__device__ __forceinline__ void y2_inv_reg(cplx<T>* v, cplx<T>* P, int zs, const cplx<T>* tw2) {
+    using Y = Y2Reg<NY, NZ>;
+    const bool act = threadIdx.x < Y::NTH;
+    const int c = threadIdx.x % NZ, b = threadIdx.x / NZ;
+    __syncthreads();  // the staged twiddles are published
+    if (act) {
+        gpe::ct::Dft<T, Y::A, +1>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < Y::A; ++k1) v[k1] = gpe::cmulc(v[k1], tw2[(b * k1) % Y::L]);
+#pragma unroll
+        for (int k1 = 0; k1 < Y::A; ++k1) P[(k1 * Y::B + b) * Y::ES + c] = v[k1];
+    }
+    __syncthreads();
+    cplx<T> w[Y::R][Y::B];
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < Y::R; ++r) {
+            const int k1 = b + Y::B * r;
+#pragma unroll
+            for (int bb = 0; bb < Y::B; ++bb) w[r][bb] = P[(k1 * Y::B + bb) * Y::ES + c];
+            gpe::ct::Dft<T, Y::B, +1>::run(w[r]);
+        }
+    }
+    __syncthreads();  // every exchange read is done before the packed rows overwrite it
+    if (act) {
+        const int kz = c + 1;
+#pragma unroll
+        for (int r = 0; r < Y::R; ++r) {
+            const int k1 = b + Y::B * r;
+#pragma unroll
+            for (int k2 = 0; k2 < Y::B / 4; ++k2) {
+                const int j = k1 + 16 * k2;  // j < NY/2; its partner j + NY/2 is output k2 + B/4
+                const cplx<T> xa = w[r][k2], xb = w[r][k2 + Y::B / 4];
+                if (kz == NZ) {
+                    P[j * zs + NZ] = gpe::mk(xa.x, xb.x);
+                } else {
+                    P[j * zs + kz] = gpe::mk(xa.x - xb.y, xa.y + xb.x);
+                    P[j * zs + (2 * NZ - kz)] = gpe::mk(xa.x + xb.y, xb.x - xa.y);
+                }
+                if (c == 0) P[j * zs] = gpe::mk(T(0), T(0));
+            }
+        }
+    }
+}
+
+// Forward y after the packed z R2C (P rows m < NY/2 hold rho_m + i rho_m+NY/2
+// transformed along z): separates the two spectra per element, transforms
+// the column (rows y >= NY are the zero padding) and stores S1 of plane i
+// straight from registers (s1_plane = S1 + i NZ, row stride nx NZ).
+template <typename T, int NY, int NZ>
+__device__ __forceinline__ void y2_fwd_reg_store(cplx<T>* P, int zs, const cplx<T>* tw2, cplx<T>* s1_plane, int nx) {
+    using Y = Y2Reg<NY, NZ>;
+    const bool act = threadIdx.x < Y::NTH;
+    const int c = threadIdx.x % NZ, b = threadIdx.x / NZ;
+    const int kz = c + 1;
+    cplx<T> v[Y::A];
+    if (act) {
+#pragma unroll
+        for (int a = 0; a < Y::A; ++a) {
+            v[a] = gpe::mk(T(0), T(0));
+            if (a < 8) {  // y = B a + b < NY
+                const int y = Y::B * a + b;
+                const int m = a < 4 ? y : y - NY / 2;
+                const cplx<T> zk = P[m * zs + kz];
+                const cplx<T> zm = P[m * zs + (2 * NZ - kz)];
+                v[a] = a < 4 ? gpe::mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y))
+                             : gpe::mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
+            }
+        }
+        gpe::ct::Dft<T, Y::A, -1>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < Y::A; ++k1) v[k1] = gpe::cmul(v[k1], tw2[(b * k1) % Y::L]);
+    }
+    __syncthreads();  // the z spectra are read: the exchange may overwrite them
+    if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < Y::A; ++k1) P[(k1 * Y::B + b) * Y::ES + c] = v[k1];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < Y::R; ++r) {
+            const int k1 = b + Y::B * r;
+            cplx<T> w[Y::B];
+#pragma unroll
+            for (int bb = 0; bb < Y::B; ++bb) w[bb] = P[(k1 * Y::B + bb) * Y::ES + c];
+            gpe::ct::Dft<T, Y::B, -1>::run(w);
+#pragma unroll
+            for (int k2 = 0; k2 < Y::B; ++k2) s1_plane[(size_t)(k1 + 16 * k2) * nx * NZ + c] = w[k2];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Plane kernel.  Dynamic LDS: the plane tile, max(2ny*(nz+1), ny*(2nz+1))
 // complex T, followed for the compiled shapes by the staged tables: the
 // twiddle table (tw_n entries), ez (nz) and ey (ny).
@@ -326,16 +588,27 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
     const cplx<T>* tw = a.tw;
     const cplx<T>* ez = a.ez;
     const cplx<T>* ey = a.ey;
-    constexpr int R2 = kStage ? Pol::template rq2<NT>() : 1;
-    cplx<T> st[R2];  // S1 plane (Poisson finish input)
+    constexpr bool kRegY2 = Pol::kRegY2;
+    constexpr int R2 = (kStage && !kRegY2) ? Pol::template rq2<NT>() : 1;
+    cplx<T> st[R2];        // S1 plane (Poisson finish input), element order
+    cplx<T> sy[kRegY2 ? 16 : 1];  // S1 column of the register y pass: S1[B a + b][i][c]
     cplx<T> pre[PRE];
     T vpre[PRE];
     if constexpr (kStage) {
         if (mode & M_PINV) {
+            if constexpr (kRegY2) {
+                constexpr int B = Y2Reg<Pol::kNY, Pol::kNZ>::B;
+                if (threadIdx.x < Y2Reg<Pol::kNY, Pol::kNZ>::NTH) {
+                    const int c = threadIdx.x % nz, b = threadIdx.x / nz;
 #pragma unroll
-            for (int q = 0; q < R2; ++q) {
-                const int e = threadIdx.x + q * NT;
-                if (e < ny2 * nz) st[q] = a.s1[((size_t)(e / nz) * nx + i) * nz + e % nz];
+                    for (int q = 0; q < 16; ++q) sy[q] = a.s1[((size_t)(B * q + b) * nx + i) * nz + c];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < R2; ++q) {
+                    const int e = threadIdx.x + q * NT;
+                    if (e < ny2 * nz) st[q] = a.s1[((size_t)(e / nz) * nx + i) * nz + e % nz];
+                }
             }
         }
 #pragma unroll
@@ -368,6 +641,19 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         }
     }
 
+    // phase couplings of this step (a dependent pair of loads when they come
+    // from the device table: issued here, consumed after the Poisson finish)
+    double pg = a.pg, pgm = a.pgm;
+    if (a.coef && (mode & (M_PHASE_HALF | M_PHASE_FULL))) {
+        const int t = (a.cbase ? *a.cbase : 0) + a.cidx;
+        pg = a.coef[2 * t];
+        pgm = a.coef[2 * t + 1];
+        if (mode & M_PHASE_FULL) {
+            pg += a.coef[2 * t + 2];
+            pgm += a.coef[2 * t + 3];
+        }
+    }
+
     T phi_r[PPT];
     T rho_r[PPT];
 
@@ -375,7 +661,26 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
     if (mode & M_NORM_IN) s_in = 1.0 / sqrt(sum_partials(a.part_in, nx, red));
 
     // ---- 1. Phi for this plane -------------------------------------------
-    if (mode & M_PINV) {
+    if ((mode & M_PINV) && kRegY2) {
+        const T sc = (T)(a.pinv_scale * s_in * s_in);
+        if constexpr (kRegY2) {
+            y2_inv_reg<T, Pol::kNY, Pol::kNZ>(sy, P, zs, tw + a.py2.tw);
+            __syncthreads();
+            Pol::template fft_z2<T, NT, +1>(P, ny / 2, 1, zs, a, tw);
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < npl) {
+                    const int j = p / nz, k = p - j * nz;
+                    const int hj = j >= ny / 2;
+                    const cplx<T> v = P[(j - hj * (ny / 2)) * zs + k];
+                    phi_r[q] = (hj ? v.y : v.x) * sc;
+                    if (mode & M_PHI_STORE) a.phi[plane_off + p] = phi_r[q];
+                }
+            }
+        }
+        __syncthreads();
+    } else if (mode & M_PINV) {
         // load S1[ky][i][kz-1] -> P[ky][kz], kz = 1..nz; the kz = 0 column
         // of rho^ Vdd is identically zero (Vdd = kz^2/k^2) and is not stored
         if constexpr (kStage) {
@@ -396,7 +701,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         const T sc = (T)(a.pinv_scale * s_in * s_in);
         if constexpr (kStage) {
             // z C2R, two rows per complex transform (ny is even for the
-            // compiled shapes): row m carries X_2m + i X_2m+1, each
+            // compiled shapes): row m carries X_m + i X_m+ny/2, each
             // Hermitian-extended with real DC and Nyquist terms, so the real
             // and imaginary parts of its inverse are the two real rows
             constexpr int RQH = (RQ + 1) / 2;
@@ -407,8 +712,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 const int w = threadIdx.x + q * NT;
                 if (w < nw) {
                     const int m = w / nzh, kz = w - m * nzh;
-                    xa[q] = P[(2 * m) * nzh + kz];
-                    xb[q] = P[(2 * m + 1) * nzh + kz];
+                    xa[q] = P[m * nzh + kz];
+                    xb[q] = P[(m + ny / 2) * nzh + kz];
                 }
             }
             __syncthreads();
@@ -432,8 +737,9 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 const int p = threadIdx.x + q * NT;
                 if (p < npl) {
                     const int j = p / nz, k = p - j * nz;
-                    const cplx<T> v = P[(j >> 1) * zs + k];
-                    phi_r[q] = ((j & 1) ? v.y : v.x) * sc;
+                    const int hj = j >= ny / 2;
+                    const cplx<T> v = P[(j - hj * (ny / 2)) * zs + k];
+                    phi_r[q] = (hj ? v.y : v.x) * sc;
                     if (mode & M_PHI_STORE) a.phi[plane_off + p] = phi_r[q];
                 }
             }
@@ -558,16 +864,6 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         }
         __syncthreads();
         const bool iint = (i >= 1 && i <= nx - 2);
-        double pg = a.pg, pgm = a.pgm;
-        if (a.coef) {
-            const int t = (a.cbase ? *a.cbase : 0) + a.cidx;
-            pg = a.coef[2 * t];
-            pgm = a.coef[2 * t + 1];
-            if (mode & M_PHASE_FULL) {
-                pg += a.coef[2 * t + 2];
-                pgm += a.coef[2 * t + 3];
-            }
-        }
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int p = threadIdx.x + q * NT;
@@ -578,16 +874,14 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 if (mode & (M_PHASE_HALF | M_PHASE_FULL)) {
                     const int j = p / nz, k = p - j * nz;
                     if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
-                        // phase argument in FP64 and reduced to [-pi, pi] before
-                        // the sincos of the bulk precision, so a large phase
-                        // (dense states, long steps) keeps its absolute accuracy
+                        // phase argument in FP64, reduced in FP64 before the
+                        // sin / cos of the bulk precision, so a large phase (dense
+                        // states, long steps) keeps its absolute accuracy
                         const double dn = (double)nrm;
                         const double vv = kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
-                        const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * dn * sqrt(dn);
-                        double th = -a.hdt * veff;
-                        th -= 6.283185307179586 * rint(th * 0.15915494309189535);
+                        const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
                         T sn, cs;
-                        sincos_t<T>((T)th, &sn, &cs);
+                        gpe::sincos_phase(-a.hdt * veff, &sn, &cs);
                         c = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
                         Q[qi] = c;
                         nrm = c.x * c.x + c.y * c.y;
@@ -602,14 +896,26 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
             for (int p = threadIdx.x; p < npl; p += NT) a.out[plane_off + p] = Q[(p / nz) * qs + p % nz];
         }
         if (mode & M_KIN) {
+            if (mode & M_OUT) __syncthreads();  // the psi output above has read Q
             // z propagation: rows j, length nz
-            Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a, tw);
-            for (int p = threadIdx.x; p < npl; p += NT) {
-                const int qi = (p / nz) * qs + p % nz;
-                Q[qi] = gpe::cmul(Q[qi], ez[p % nz]);
+            if constexpr (Pol::kRegZ) {
+                kin_z_reg<T, Pol::kNZ, NT>(Q, ny, qs, ez, tw + a.pz.tw);
+                __syncthreads();
+            } else {
+                Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a, tw);
+                for (int p = threadIdx.x; p < npl; p += NT) {
+                    const int qi = (p / nz) * qs + p % nz;
+                    Q[qi] = gpe::cmul(Q[qi], ez[p % nz]);
+                }
+                __syncthreads();
+                Pol::template fft_z<T, NT, +1>(Q, ny, 1, qs, a, tw);
             }
-            __syncthreads();
-            Pol::template fft_z<T, NT, +1>(Q, ny, 1, qs, a, tw);
+        }
+        if ((mode & M_KIN) && Pol::kRegY) {
+            // y propagation in registers, result straight to Tk
+            if constexpr (Pol::kRegY)
+                kin_y_reg_store<T, Pol::kNY, Pol::kNZ, NT>(Q, qs, ey, tw + a.py.tw, a.tko + (size_t)i * nz, nx);
+        } else if (mode & M_KIN) {
             // y propagation: columns k, length ny
             Pol::template fft_y<T, NT, -1>(Q, nz, qs, 1, a, tw);
             for (int p = threadIdx.x; p < npl; p += NT) {
@@ -635,7 +941,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
     if (mode & M_RHO_FWD) {
         cplx<T> tmp[RQ];
         if constexpr (kStage) {
-            // z R2C, two rows per complex transform: row m = rho_2m + i rho_2m+1
+            // z R2C, two rows per complex transform: row m = rho_m + i rho_m+ny/2
             // (zero for k in [nz, 2nz)); the spectra separate as
             // A = (Z_k + conj Z_-k) / 2, B = (Z_k - conj Z_-k) / 2i
 #pragma unroll
@@ -643,20 +949,27 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 const int p = threadIdx.x + q * NT;
                 if (p < npl) {
                     const int j = p / nz, k = p - j * nz;
-                    reinterpret_cast<T*>(&P[(j >> 1) * zs + k])[j & 1] = rho_r[q];
-                    if (!(j & 1)) P[(j >> 1) * zs + nz + k] = gpe::mk(T(0), T(0));
+                    const int hj = j >= ny / 2;
+                    reinterpret_cast<T*>(&P[(j - hj * (ny / 2)) * zs + k])[hj] = rho_r[q];
+                    if (!hj) P[j * zs + nz + k] = gpe::mk(T(0), T(0));
                 }
             }
             __syncthreads();
             Pol::template fft_z2<T, NT, -1>(P, ny / 2, 1, zs, a, tw);
+            if constexpr (kRegY2) {
+                // y in registers, S1 stored from registers: the plane is done
+                y2_fwd_reg_store<T, Pol::kNY, Pol::kNZ>(P, zs, tw + a.py2.tw, a.s1o + (size_t)i * nz, nx);
+                return;
+            }
 #pragma unroll
             for (int q = 0; q < RQ; ++q) {
                 const int e = threadIdx.x + q * NT;
                 if (e < ny * nzh) {
                     const int j = e / nzh, kz = e - j * nzh;
-                    const cplx<T> zk = P[(j >> 1) * zs + kz];
-                    const cplx<T> zm = P[(j >> 1) * zs + (kz == 0 ? 0 : nz2 - kz)];
-                    if (j & 1)
+                    const int m = j >= ny / 2 ? j - ny / 2 : j;
+                    const cplx<T> zk = P[m * zs + kz];
+                    const cplx<T> zm = P[m * zs + (kz == 0 ? 0 : nz2 - kz)];
+                    if (j >= ny / 2)
                         tmp[q] = gpe::mk(T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x));
                     else
                         tmp[q] = gpe::mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
@@ -907,16 +1220,18 @@ struct XReg {
 };
 
 // LDS: exchange area (16 * 256 + 256 complex) + tables (2 * 2NX complex).
+// The kinetic tiles (twice the loads of a zero-padded Poisson tile) take the
+// first block indices so that they are dispatched first.
 template <typename T, int NX>
-__global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_p) {
+__global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_k) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
     cplx<T>* TB = E + 16 * 256 + 256;  // exchange area: L * (LPB + 1) <= 4096 + 256 elements
     using X = XReg<NX>;
-    if ((int)blockIdx.x < nblk_p)
-        xline_reg<T, X::LP, X::LPB_P, true>(a, blockIdx.x, E, TB);
+    if ((int)blockIdx.x < nblk_k)
+        xline_reg<T, X::LK, X::LPB_K, false>(a, blockIdx.x, E, TB);
     else
-        xline_reg<T, X::LK, X::LPB_K, false>(a, blockIdx.x - nblk_p, E, TB);
+        xline_reg<T, X::LP, X::LPB_P, true>(a, blockIdx.x - nblk_k, E, TB);
 }
 
 // Batched launches: one block per work item (instance, plane) / (instance,
@@ -1002,12 +1317,10 @@ __global__ void k_potential_half(cplx<T>* psi, const T* V, const T* phi, int nx,
         const int i = e / ((size_t)ny * nz);
         if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > nz - 2) continue;
         cplx<T> c = psi[e];
-        const double nrm = (double)(c.x * c.x + c.y * c.y);
-        const double veff = (double)V[e] + cdd * (double)phi[e] + pg * nrm + pgm * nrm * sqrt(nrm);
-        double th = -hdt * veff;
-        th -= 6.283185307179586 * rint(th * 0.15915494309189535);
+        const T nr = c.x * c.x + c.y * c.y;
+        const double veff = (double)V[e] + cdd * (double)phi[e] + pg * (double)nr + pgm * rho32(nr);
         T sn, cs;
-        sincos_t<T>((T)th, &sn, &cs);
+        gpe::sincos_phase(-hdt * veff, &sn, &cs);
         psi[e] = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
     }
 }
@@ -1456,7 +1769,7 @@ static int launch_line_reg(gpe_handle* h, const LineArgs<T>& a, bool poisson, bo
     const size_t lds = (size_t)(16 * 256 + 256 + 4 * NX) * sizeof(cplx<T>);
     hipEvent_t e0, e1;
     kernel_events(h, 1, &e0, &e1);
-    hipExtLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, e0, e1, 0, a, np);
+    hipExtLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, e0, e1, 0, a, nk);
     GPE_CHECK(hipGetLastError());
     return 0;
 }
