@@ -82,13 +82,17 @@ double potential_value(const std::string& key, const PhysicalParameters& p, doub
     throw std::runtime_error("Not know potential option");
 }
 
+// initializer.cpp:229-248.  The reference samples the potential through a
+// std::function<double(int, int, int)> (initializer.hpp:37): the coordinates
+// are truncated toward zero to whole atomic units on the way in.
 void fill_potential(const std::string& key, const PhysicalParameters& p, potential_t& V) {
     V.assign((size_t)p.nx * p.ny * p.nz, 0.0);
+    auto t = [](double c) { return (double)(int)c; };
     for (int i = 0; i < p.nx; i++)
         for (int j = 0; j < p.ny; j++)
             for (int k = 0; k < p.nz; k++)
-                V[((size_t)i * p.ny + j) * p.nz + k] =
-                    potential_value(key, p, coord(p.nx, p.dx, i), coord(p.ny, p.dy, j), coord(p.nz, p.dz, k));
+                V[((size_t)i * p.ny + j) * p.nz + k] = potential_value(
+                    key, p, t(coord(p.nx, p.dx, i)), t(coord(p.ny, p.dy, j)), t(coord(p.nz, p.dz, k)));
 }
 
 void init_wavefunction(const PhysicalParameters& p, wavefunction_t& psi) {

@@ -155,10 +155,13 @@ def coords(n: int, d: float) -> np.ndarray:
 
 
 def potential(key: str, p: PhysicalParameters) -> np.ndarray:
-    """V sampled on every grid point (initializer.cpp:229-248, potentials.hpp:76-157)."""
-    x = coords(p.nx, p.dx)[:, None, None]
-    y = coords(p.ny, p.dy)[None, :, None]
-    z = coords(p.nz, p.dz)[None, None, :]
+    """V sampled on every grid point (initializer.cpp:229-248, potentials.hpp:76-157).
+    The reference samples through a std::function<double(int, int, int)>
+    (initializer.hpp:37), so the coordinates are truncated toward zero to
+    whole atomic units before the potential sees them."""
+    x = np.trunc(coords(p.nx, p.dx))[:, None, None]
+    y = np.trunc(coords(p.ny, p.dy))[None, :, None]
+    z = np.trunc(coords(p.nz, p.dz))[None, None, :]
     vy = 0.5 * p.m * y**2 * p.omega_y**2
     vz = 0.5 * p.m * z**2 * p.omega_z**2
     slope = (0.25 / (27211.4 * 11.6 * 1e9)) / p.dd

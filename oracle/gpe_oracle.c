@@ -84,14 +84,20 @@ double orc_potential_value(int key, const orc_params *p, double x, double y, dou
     }
 }
 
-/* src/initializer/initializer.cpp:229-248 (init_pote): V sampled on every point. */
+/* src/initializer/initializer.cpp:229-248 (init_pote): V sampled on every point.
+ * The sampler goes through DataInitializer::_pote_func, declared
+ * std::function<double(int, int, int)> (src/initializer/initializer.hpp:37):
+ * the coordinates reach the potential truncated toward zero to whole atomic
+ * units of length. */
+static double orc_trunc_coord(double x) { return (double)(int)x; }
 void orc_fill_potential(int key, const orc_params *p, double *V) {
     for (int i = 0; i < p->nx; i++)
         for (int j = 0; j < p->ny; j++)
             for (int k = 0; k < p->nz; k++)
                 V[((size_t)i * p->ny + j) * p->nz + k] =
-                    orc_potential_value(key, p, orc_coord(p->nx, p->dx, i),
-                                        orc_coord(p->ny, p->dy, j), orc_coord(p->nz, p->dz, k));
+                    orc_potential_value(key, p, orc_trunc_coord(orc_coord(p->nx, p->dx, i)),
+                                        orc_trunc_coord(orc_coord(p->ny, p->dy, j)),
+                                        orc_trunc_coord(orc_coord(p->nz, p->dz, k)));
 }
 
 /* ---------------------------------------------------------- initializer */
