@@ -30,7 +30,7 @@ EXPORTS = (
     "gpe_gemm_test",
 )
 # Every symbol include/gpesolver_host.h declares.
-HOST_EXPORTS = ("gpes_validate_params_json", "gpes_run")
+HOST_EXPORTS = ("gpes_validate_params_json", "gpes_run", "gpes_write_files")
 
 
 class GpeParams(C.Structure):
@@ -111,6 +111,8 @@ def load(path: str = LIB_PATH):
     L.gpes_validate_params_json.restype = ip
     L.gpes_run.argtypes = [C.c_char_p, C.c_double, dp, C.POINTER(C.c_long), C.c_char_p, ip]
     L.gpes_run.restype = ip
+    L.gpes_write_files.argtypes = [C.c_char_p, C.c_char_p, dp, C.c_char_p, dp, dp, ip, C.c_char_p, ip]
+    L.gpes_write_files.restype = ip
     _lib = L
     return L
 
@@ -297,3 +299,17 @@ class Batch:
         if rc != 0:
             raise GpeError(self.L.gpe_batch_last_error(self.b).decode())
         return np.array(steps[:], dtype=np.int64), e
+
+
+def write_files(workdir: str, name: str, psi: np.ndarray, pote_name: str, V: np.ndarray, energies: np.ndarray):
+    """The reference's on-disk formats (gpes_write_files) for the grid of
+    workdir/gpe_params.json; energies is [n, 5] (kin, pot, int, ext, bmf)."""
+    L = load()
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    en = np.ascontiguousarray(energies, dtype=np.float64).reshape(-1, 5)
+    err = C.create_string_buffer(512)
+    rc = L.gpes_write_files(workdir.encode(), name.encode(), _dp(psi.view(np.float64)), pote_name.encode(), _dp(V),
+                            _dp(en), en.shape[0], err, 512)
+    if rc != 0:
+        raise GpeError(err.value.decode())

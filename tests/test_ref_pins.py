@@ -168,3 +168,27 @@ def test_loader_accepts_and_rejects_like_the_reference(tmp_path, v):
     ref_ok = out.strip().splitlines()[-1] == "ok"
     ours_err = hip.validate_params_json(json.dumps(cfg))
     assert (ours_err is None) == ref_ok, (out, ours_err)
+
+
+def test_file_formats_byte_for_byte(tmp_path):
+    """Checkpoint (.gpe.bin), state and potential text files (.gpe.dat) and
+    energy.gpe.dat written by the host layer equal the reference's own writers'
+    output for the same data (file_manager.cpp:81-112, 251-288, 387-434)."""
+    cfg = dict(BASE, init_strategy="GAUSS", pote_strategy="MEXICAN", nx=6, ny=5, nz=4)
+    d, _ = run_ref(tmp_path, cfg, mode="validate")
+    rng = np.random.default_rng(11)
+    n = 6 * 5 * 4
+    psi = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) * 1e-3
+    V = rng.standard_normal(n) * 1e-13
+    psi.tofile(d / "in_psi.raw")
+    V.tofile(d / "in_pote.raw")
+    r = subprocess.run([DRIVER, str(d), "save"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    ours_dir = tmp_path / "ours"
+    ours_dir.mkdir()
+    (ours_dir / "gpe_params.json").write_text(json.dumps(cfg))
+    t = np.arange(3)
+    en = np.stack([1.25e-13 * (t + 1), 2.5e-14 / (t + 1), -3.0e-15 * t, 4.0e-16 + 0 * t, 5.0e-17 * t], axis=1)
+    hip.write_files(str(ours_dir), "ref_state", psi, "ref_pote", V, en)
+    for f in ("ref_state.gpe.bin", "ref_state.gpe.dat", "ref_pote.gpe.dat", "energy.gpe.dat"):
+        assert (ours_dir / f).read_bytes() == (d / f).read_bytes(), f
