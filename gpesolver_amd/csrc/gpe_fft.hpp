@@ -4,7 +4,7 @@
 // block.  Each Stockham step loads every butterfly's inputs of the whole tile
 // into registers, passes a barrier, then writes the butterfly outputs to their
 // self-sorted positions, so one LDS buffer suffices (no ping-pong).  Radices
-// 4, 2 and 3 cover every grid length of the solver (powers of two and 3*2^k).
+// 4, 2, 3, 5 and 7 cover every length 2^a 3^b 5^c 7^d.
 //
 // Step with radix R over a line of length L, sub-transform size Ns:
 //   butterfly j in [0, L/R):  k = j mod Ns
@@ -82,6 +82,41 @@ __device__ __forceinline__ void dft_small(cplx<T> (&a)[R], int sign) {
         a[0] = cadd(a[0], s);
         a[1] = cadd(m, e);
         a[2] = csub(m, e);
+    } else if constexpr (R == 5 || R == 7) {
+        // direct DFT with the R-th roots of unity (cos / sin of 2 pi m / R)
+        constexpr double c5[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
+                                  0.30901699437494742410};
+        constexpr double s5[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
+                                  -0.95105651629515357212};
+        constexpr double c7[7] = {1.0,
+                                  0.62348980185873353053,
+                                  -0.22252093395631440429,
+                                  -0.90096886790241912624,
+                                  -0.90096886790241912624,
+                                  -0.22252093395631440429,
+                                  0.62348980185873353053};
+        constexpr double s7[7] = {0.0,
+                                  0.78183148246802980871,
+                                  0.97492791218182360702,
+                                  0.43388373911755812048,
+                                  -0.43388373911755812048,
+                                  -0.97492791218182360702,
+                                  -0.78183148246802980871};
+        cplx<T> out[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            cplx<T> acc = a[0];
+#pragma unroll
+            for (int t = 1; t < R; ++t) {
+                const int m = (q * t) % R;
+                const T c = (T)(R == 5 ? c5[m] : c7[m]);
+                const T sn = (T)(sign * (R == 5 ? s5[m] : s7[m]));  // W^(sign m) = c + i sign s
+                acc = cadd(acc, mk(a[t].x * c - a[t].y * sn, a[t].x * sn + a[t].y * c));
+            }
+            out[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) a[q] = out[q];
     }
 }
 
@@ -147,8 +182,12 @@ __device__ void lds_fft(cplx<T>* buf, const LinePlan& pl, int nlines, int es, in
             stockham_step<T, NT, MAXE, 4>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
         else if (R == 2)
             stockham_step<T, NT, MAXE, 2>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
-        else
+        else if (R == 3)
             stockham_step<T, NT, MAXE, 3>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
+        else if (R == 5)
+            stockham_step<T, NT, MAXE, 5>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
+        else
+            stockham_step<T, NT, MAXE, 7>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
         Ns *= R;
     }
 }

@@ -1525,6 +1525,14 @@ static bool make_plan(int L, LinePlan& pl) {
         pl.radix[pl.nst++] = 3;
         r /= 3;
     }
+    while (r % 5 == 0 && pl.nst < gpe::kMaxStages) {
+        pl.radix[pl.nst++] = 5;
+        r /= 5;
+    }
+    while (r % 7 == 0 && pl.nst < gpe::kMaxStages) {
+        pl.radix[pl.nst++] = 7;
+        r /= 7;
+    }
     return r == 1;
 }
 
@@ -1582,7 +1590,7 @@ static int setup_tables(gpe_handle* h) {
     std::vector<cplx<T>> tw;
     for (int t = 0; t < 6; ++t) {
         if (!make_plan(lens[t], *pls[t])) return fail(h, "grid length " + std::to_string(lens[t]) +
-                                                             " is not of the form 2^a 3^b");
+                                                             " is not of the form 2^a 3^b 5^c 7^d");
         pls[t]->tw = (int)tw.size();
         for (int m = 0; m < lens[t]; ++m) {
             const double ang = 2.0 * M_PI * m / lens[t];

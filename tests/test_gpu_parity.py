@@ -259,10 +259,12 @@ def test_deterministic_rerun(oracle):
     assert outs[0][1] == outs[1][1]
 
 
-@pytest.mark.parametrize("shape,force", [((36, 18, 12), "0"), ((32, 24, 12), "1"), ((16, 16, 8), "1")])
+@pytest.mark.parametrize("shape,force", [((36, 18, 12), "0"), ((32, 24, 12), "1"), ((16, 16, 8), "1"),
+                                         ((20, 10, 6), "0"), ((28, 14, 10), "0")])
 def test_generic_runtime_size_kernels(oracle, shape, force, monkeypatch):
-    """Grids without compile-time kernels (36x18x12), and GPE_FORCE_GENERIC=1
-    on compiled shapes, run the run-time-size radix-4/2/3 engine."""
+    """Grids without compile-time kernels (36x18x12; 20x10x6 and 28x14x10 with
+    radix-5 and radix-7 lengths), and GPE_FORCE_GENERIC=1 on compiled shapes,
+    run the run-time-size radix-4/2/3/5/7 engine."""
     monkeypatch.setenv("GPE_FORCE_GENERIC", force)
     for prec in (hip.GPE_FP64, hip.GPE_FP32):
         p, dev, orc = make_pair(oracle, shape, prec, pot="HARMONIC", init="GAUSS", n_atoms=500.0)

@@ -1,0 +1,32 @@
+"""Learned warm start end to end on the GPU: ground states generated with the
+batched IT solver, an FNO trained on them with the project's own kernels, and
+IT from the FNO's prediction against plain MULTIPLE_GAUSS on held-out
+instances.  A small network and training set, so the test runs in seconds;
+the full-size experiment is gpesolver_amd.warmstart_experiment."""
+import numpy as np
+import pytest
+
+from gpesolver_amd import fno as F
+from gpesolver_amd import warmstart as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_warm_start_needs_fewer_it_steps():
+    rng = np.random.default_rng(7)
+    insts = W.sample_instances(rng, 40)
+    train, test = insts[:32], insts[32:]
+    X, T, S = W.generate(train, tol=1e-6, max_steps=5000, log=lambda m: print(m, flush=True))
+    model = F.Fno(*W.GRID, width=16, layers=2, hidden=32, modes=8, max_batch=8, seed=3)
+    hist = W.train(model, X, T, epochs=30, lr=2e-3, batch=8, log=lambda m: None)
+    print("train mse first / last", hist[0], hist[-1], flush=True)
+    assert hist[-1] < 0.5 * hist[0]
+    res = W.compare(model, test, tol=1e-6, max_steps=5000)
+    model.close()
+    print({k: res[k] for k in ("plain_steps_mean", "warm_steps_mean", "plain_wall_s", "warm_wall_s",
+                               "fallbacks", "energy_rel_diff_median")}, flush=True)
+    assert res["fallbacks"] == 0
+    assert res["warm_steps_mean"] < res["plain_steps_mean"]
+    # the warm run reaches the plain run's energy (or a lower one)
+    pe, we = np.array(res["plain_e_per_atom"]), np.array(res["warm_e_per_atom"])
+    assert np.all(we <= pe + 1e-4 * np.abs(pe))
