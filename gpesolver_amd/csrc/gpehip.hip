@@ -49,6 +49,10 @@
 using gpe::cplx;
 using gpe::LinePlan;
 
+// Reciprocal: the hardware v_rcp_f32 (1 ulp) in FP32, a division in FP64.
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
+
 // Streaming (nontemporal) store of one complex value.
 template <typename T>
 __device__ __forceinline__ void store_nt(cplx<T>* p, cplx<T> v) {
@@ -899,7 +903,9 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
             if (mode & M_OUT) __syncthreads();  // the psi output above has read Q
             // z propagation: rows j, length nz
             if constexpr (Pol::kRegZ) {
-                kin_z_reg<T, Pol::kNZ, NT>(Q, ny, qs, ez, tw + a.pz.tw);
+                // tables named through the LDS base, so their reads compile to LDS loads
+                const cplx<T>* TWs = P + max(ny2 * nzh, ny * zs);
+                kin_z_reg<T, Pol::kNZ, NT>(Q, ny, qs, TWs + a.tw_n, TWs + a.pz.tw);
                 __syncthreads();
             } else {
                 Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a, tw);
@@ -913,8 +919,11 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         }
         if ((mode & M_KIN) && Pol::kRegY) {
             // y propagation in registers, result straight to Tk
-            if constexpr (Pol::kRegY)
-                kin_y_reg_store<T, Pol::kNY, Pol::kNZ, NT>(Q, qs, ey, tw + a.py.tw, a.tko + (size_t)i * nz, nx);
+            if constexpr (Pol::kRegY) {
+                const cplx<T>* TWs = P + max(ny2 * nzh, ny * zs);
+                kin_y_reg_store<T, Pol::kNY, Pol::kNZ, NT>(Q, qs, TWs + a.tw_n + nz, TWs + a.py.tw,
+                                                           a.tko + (size_t)i * nz, nx);
+            }
         } else if (mode & M_KIN) {
             // y propagation: columns k, length ny
             Pol::template fft_y<T, NT, -1>(Q, nz, qs, 1, a, tw);
@@ -1181,7 +1190,7 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
 #pragma unroll
             for (int k2 = 0; k2 < B; ++k2) {
                 const T k2s = kx2s[k1 + A * k2] + ky2 + kz2;
-                const T vd = (k2s > T(1e-30)) ? kz2 / k2s : T(0);
+                const T vd = (k2s > T(1e-30)) ? kz2 * rcp_t(k2s) : T(0);
                 w[r][k2] = gpe::cscale(w[r][k2], vd);
             }
         } else {
