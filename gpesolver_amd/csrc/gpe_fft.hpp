@@ -45,6 +45,67 @@ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
 template <typename T>
 __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return mk(a.x * s, a.y * s); }
 
+// Products with FP64 constants.  In FP32 the constant is split into hi + lo
+// FP32 parts, so the product carries one rounding of the exact value instead
+// of the rounding of the constant: rounded to nearest, the FP32 values of
+// cos / sin of pi/8, pi/6 and pi/4 all leave |w| below 1 by 2-6e-8, a
+// magnitude loss that every transform pass using them would add to the norm.
+template <typename T>
+__device__ __forceinline__ T mulk(T x, double k) {
+    if constexpr (sizeof(T) == 4) {
+        const float hi = (float)k;
+        const float lo = (float)(k - (double)hi);
+        if (lo == 0.0f) return x * hi;
+        return fmaf(x, hi, x * lo);
+    } else {
+        return x * (T)k;
+    }
+}
+// a * (c + i s)
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulk(cplx<T> a, double c, double s) {
+    if constexpr (sizeof(T) == 4) {
+        const float ch = (float)c, cl = (float)(c - (double)ch);
+        const float sh = (float)s, sl = (float)(s - (double)sh);
+        const float re_lo = fmaf(a.x, cl, -(a.y * sl));
+        const float im_lo = fmaf(a.x, sl, a.y * cl);
+        return mk(fmaf(a.x, ch, fmaf(-a.y, sh, re_lo)), fmaf(a.x, sh, fmaf(a.y, ch, im_lo)));
+    } else {
+        return mk(a.x * (T)c - a.y * (T)s, a.x * (T)s + a.y * (T)c);
+    }
+}
+
+// a * (h + l) and a * conj(h + l) for a twiddle held as a table value h and
+// its lo part l (FP32 transforms of psi): one rounding of the exact product.
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul2(cplx<T> a, cplx<T> h, cplx<T> l) {
+    const T re_lo = fma(a.x, l.x, -(a.y * l.y));
+    const T im_lo = fma(a.x, l.y, a.y * l.x);
+    return mk(fma(a.x, h.x, fma(-a.y, h.y, re_lo)), fma(a.x, h.y, fma(a.y, h.x, im_lo)));
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulc2(cplx<T> a, cplx<T> h, cplx<T> l) {
+    const T re_lo = fma(a.x, l.x, a.y * l.y);
+    const T im_lo = fma(a.y, l.x, -(a.x * l.y));
+    return mk(fma(a.x, h.x, fma(a.y, h.y, re_lo)), fma(a.y, h.x, fma(-a.x, h.y, im_lo)));
+}
+
+// a (1 + e) k for a kinetic phase entry e = (cos f - 1, sin f) and the
+// normalisation k = 1/n.  The table holds the deviation from 1, so its FP32
+// rounding keeps |1 + e| = 1 to ~1e-9 at the small phases of the low
+// wavenumbers that hold the mass; an FP32 (cos f, sin f) pair is off by up
+// to 6e-8 there, and every real-time step would add that to the norm.  The
+// increment b e is formed first and b + b e rounded once: at small f the
+// -f^2/2 part of the increment is below half an ulp of b, and a chain of
+// roundings through b + i f b would drop it (a norm gain of f^2 per multiply).
+template <typename T>
+__device__ __forceinline__ cplx<T> kin_mul(cplx<T> a, cplx<T> e, double k) {
+    const cplx<T> b = mk(mulk(a.x, k), mulk(a.y, k));
+    const T dx = fma(b.x, e.x, -(b.y * e.y));
+    const T dy = fma(b.y, e.x, b.x * e.y);
+    return mk(b.x + dx, b.y + dy);
+}
+
 // Radix plan of one line length, stored in a small struct passed by value.
 struct LinePlan {
     int L;        // line length
@@ -73,12 +134,13 @@ __device__ __forceinline__ void dft_small(cplx<T> (&a)[R], int sign) {
         a[1] = cadd(t1, d);
         a[3] = csub(t1, d);
     } else if constexpr (R == 3) {
-        const T h = T(0.86602540378443864676);  // sqrt(3)/2
+        const double h = 0.86602540378443864676;  // sqrt(3)/2
         cplx<T> s = cadd(a[1], a[2]);
         cplx<T> d = csub(a[1], a[2]);
         cplx<T> m = mk(a[0].x - T(0.5) * s.x, a[0].y - T(0.5) * s.y);
         // e = sign * i * h * d
-        cplx<T> e = (sign < 0) ? mk(h * d.y, -h * d.x) : mk(-h * d.y, h * d.x);
+        const T hx = mulk(d.x, h), hy = mulk(d.y, h);
+        cplx<T> e = (sign < 0) ? mk(hy, -hx) : mk(-hy, hx);
         a[0] = cadd(a[0], s);
         a[1] = cadd(m, e);
         a[2] = csub(m, e);
@@ -109,9 +171,9 @@ __device__ __forceinline__ void dft_small(cplx<T> (&a)[R], int sign) {
 #pragma unroll
             for (int t = 1; t < R; ++t) {
                 const int m = (q * t) % R;
-                const T c = (T)(R == 5 ? c5[m] : c7[m]);
-                const T sn = (T)(sign * (R == 5 ? s5[m] : s7[m]));  // W^(sign m) = c + i sign s
-                acc = cadd(acc, mk(a[t].x * c - a[t].y * sn, a[t].x * sn + a[t].y * c));
+                const double c = R == 5 ? c5[m] : c7[m];
+                const double sn = sign * (R == 5 ? s5[m] : s7[m]);  // W^(sign m) = c + i sign s
+                acc = cadd(acc, cmulk(a[t], c, sn));
             }
             out[q] = acc;
         }

@@ -11,59 +11,41 @@
 namespace gpe {
 namespace ct {
 
-// cos / sin of 2 pi m / 16 and 2 pi m / 12.
+// cos / sin of 2 pi m / 32 and 2 pi m / 24.
 struct TwC {
-    static constexpr double c16[16] = {1.0,
-                                       0.92387953251128675613,
-                                       0.70710678118654752440,
-                                       0.38268343236508977173,
-                                       0.0,
-                                       -0.38268343236508977173,
-                                       -0.70710678118654752440,
-                                       -0.92387953251128675613,
-                                       -1.0,
-                                       -0.92387953251128675613,
-                                       -0.70710678118654752440,
-                                       -0.38268343236508977173,
-                                       0.0,
-                                       0.38268343236508977173,
-                                       0.70710678118654752440,
-                                       0.92387953251128675613};
-    static constexpr double s16[16] = {0.0,
-                                       0.38268343236508977173,
-                                       0.70710678118654752440,
-                                       0.92387953251128675613,
-                                       1.0,
-                                       0.92387953251128675613,
-                                       0.70710678118654752440,
-                                       0.38268343236508977173,
-                                       0.0,
-                                       -0.38268343236508977173,
-                                       -0.70710678118654752440,
-                                       -0.92387953251128675613,
-                                       -1.0,
-                                       -0.92387953251128675613,
-                                       -0.70710678118654752440,
-                                       -0.38268343236508977173};
-    static constexpr double c12[12] = {1.0,  0.86602540378443864676,  0.5,  0.0, -0.5, -0.86602540378443864676,
-                                       -1.0, -0.86602540378443864676, -0.5, 0.0, 0.5,  0.86602540378443864676};
-    static constexpr double s12[12] = {0.0, 0.5,  0.86602540378443864676,  1.0,  0.86602540378443864676,  0.5,
-                                       0.0, -0.5, -0.86602540378443864676, -1.0, -0.86602540378443864676, -0.5};
+    static constexpr double c32[32] = {
+        1.0, 0.980785280403230449126, 0.923879532511286756128, 0.831469612302545237079,
+        0.707106781186547524401, 0.555570233019602224743, 0.382683432365089771728, 0.195090322016128267848,
+        0.0, -0.195090322016128267848, -0.382683432365089771728, -0.555570233019602224743,
+        -0.707106781186547524401, -0.831469612302545237079, -0.923879532511286756128, -0.980785280403230449126,
+        -1.0, -0.980785280403230449126, -0.923879532511286756128, -0.831469612302545237079,
+        -0.707106781186547524401, -0.555570233019602224743, -0.382683432365089771728, -0.195090322016128267848,
+        0.0, 0.195090322016128267848, 0.382683432365089771728, 0.555570233019602224743,
+        0.707106781186547524401, 0.831469612302545237079, 0.923879532511286756128, 0.980785280403230449126};
+    static constexpr double s32[32] = {
+        0.0, 0.195090322016128267848, 0.382683432365089771728, 0.555570233019602224743,
+        0.707106781186547524401, 0.831469612302545237079, 0.923879532511286756128, 0.980785280403230449126,
+        1.0, 0.980785280403230449126, 0.923879532511286756128, 0.831469612302545237079,
+        0.707106781186547524401, 0.555570233019602224743, 0.382683432365089771728, 0.195090322016128267848,
+        0.0, -0.195090322016128267848, -0.382683432365089771728, -0.555570233019602224743,
+        -0.707106781186547524401, -0.831469612302545237079, -0.923879532511286756128, -0.980785280403230449126,
+        -1.0, -0.980785280403230449126, -0.923879532511286756128, -0.831469612302545237079,
+        -0.707106781186547524401, -0.555570233019602224743, -0.382683432365089771728, -0.195090322016128267848};
+    static constexpr double c24[24] = {
+        1.0, 0.96592582628906828675, 0.866025403784438646764, 0.707106781186547524401,
+        0.5, 0.258819045102520762349, 0.0, -0.258819045102520762349,
+        -0.5, -0.707106781186547524401, -0.866025403784438646764, -0.96592582628906828675,
+        -1.0, -0.96592582628906828675, -0.866025403784438646764, -0.707106781186547524401,
+        -0.5, -0.258819045102520762349, 0.0, 0.258819045102520762349,
+        0.5, 0.707106781186547524401, 0.866025403784438646764, 0.96592582628906828675};
+    static constexpr double s24[24] = {
+        0.0, 0.258819045102520762349, 0.5, 0.707106781186547524401,
+        0.866025403784438646764, 0.96592582628906828675, 1.0, 0.96592582628906828675,
+        0.866025403784438646764, 0.707106781186547524401, 0.5, 0.258819045102520762349,
+        0.0, -0.258819045102520762349, -0.5, -0.707106781186547524401,
+        -0.866025403784438646764, -0.96592582628906828675, -1.0, -0.96592582628906828675,
+        -0.866025403784438646764, -0.707106781186547524401, -0.5, -0.258819045102520762349};
 };
-
-// W_N^m with sign (-1: exp(-2 pi i m/N), +1: exp(+2 pi i m/N)), N | 16 or N | 12.
-template <typename T, int N, int SIGN>
-__device__ __forceinline__ cplx<T> wconst(int m) {
-    m %= N;
-    if constexpr (16 % N == 0) {
-        const int t = m * (16 / N);
-        return mk<T>((T)TwC::c16[t], (T)(SIGN * TwC::s16[t]));
-    } else {
-        static_assert(12 % N == 0, "twiddle base");
-        const int t = m * (12 / N);
-        return mk<T>((T)TwC::c12[t], (T)(SIGN * TwC::s12[t]));
-    }
-}
 
 // multiply by SIGN * i
 template <typename T, int SIGN>
@@ -71,16 +53,64 @@ __device__ __forceinline__ cplx<T> mul_si(cplx<T> a) {
     return SIGN < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
 }
 
-template <typename T, int R, int SIGN>
+// cos / sin of 2 pi m / N from the tables (any integer m).
+template <int N>
+__device__ __forceinline__ void wtab(int m, double* c, double* s) {
+    m = ((m % N) + N) % N;
+    if constexpr (32 % N == 0) {
+        *c = TwC::c32[m * (32 / N)];
+        *s = TwC::s32[m * (32 / N)];
+    } else {
+        static_assert(24 % N == 0, "twiddle base");
+        *c = TwC::c24[m * (24 / N)];
+        *s = TwC::s24[m * (24 / N)];
+    }
+}
+
+// a * i^q (SIGN -1: a * (-i)^q)
+template <typename T, int SIGN>
+__device__ __forceinline__ cplx<T> quarter(cplx<T> a, int q) {
+    q &= 3;
+    return q == 0 ? a : q == 1 ? mul_si<T, SIGN>(a) : q == 2 ? mk(-a.x, -a.y) : mul_si<T, -SIGN>(a);
+}
+
+// a * W_N^m (W = exp(SIGN 2 pi i / N), N | 32 or N | 24).  Quarter turns are
+// exact; odd multiples of pi/4 are a sum and a difference scaled by 1/sqrt 2.
+// EX (the transforms of psi, whose norm every real-time step must keep):
+// the constants enter as hi + lo pairs (mulk, cmulk), one rounding of the
+// exact product.  The rounded cos / sin pairs of pi/8, pi/6 ... fall short
+// of |w| = 1 by 2-6e-8, and even the increment form 1 + (cos r - 1, sin r)
+// of the residual angle leaves |w| - 1 at 1-5e-9: over thousands of steps
+// either shows as a norm drift.  Otherwise the constants are rounded to T.
+template <typename T, int N, int SIGN, bool EX>
+__device__ __forceinline__ cplx<T> mul_w(cplx<T> a, int m) {
+    m %= N;
+    if ((4 * m) % N == 0) return quarter<T, SIGN>(a, 4 * m / N);
+    double c, s;
+    wtab<N>(m, &c, &s);
+    s *= SIGN;
+    constexpr double r = 0.70710678118654752440;
+    if ((8 * m) % N == 0) {
+        // c = +-r, s = +-r
+        const T re = (c > 0 ? a.x : -a.x) - (s > 0 ? a.y : -a.y);
+        const T im = (s > 0 ? a.x : -a.x) + (c > 0 ? a.y : -a.y);
+        if constexpr (EX) return mk(mulk(re, r), mulk(im, r));
+        return mk(re * (T)r, im * (T)r);
+    }
+    if constexpr (EX) return cmulk(a, c, s);
+    return cmul(a, mk((T)c, (T)s));
+}
+
+template <typename T, int R, int SIGN, bool EX = false>
 struct Dft;
 
-template <typename T, int SIGN>
-struct Dft<T, 1, SIGN> {
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 1, SIGN, EX> {
     __device__ __forceinline__ static void run(cplx<T>*) {}
 };
 
-template <typename T, int SIGN>
-struct Dft<T, 2, SIGN> {
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 2, SIGN, EX> {
     __device__ __forceinline__ static void run(cplx<T>* v) {
         cplx<T> t = v[0];
         v[0] = cadd(t, v[1]);
@@ -88,22 +118,26 @@ struct Dft<T, 2, SIGN> {
     }
 };
 
-template <typename T, int SIGN>
-struct Dft<T, 3, SIGN> {
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 3, SIGN, EX> {
     __device__ __forceinline__ static void run(cplx<T>* v) {
-        const T h = T(0.86602540378443864676);
+        constexpr double h = 0.86602540378443864676;
         cplx<T> s = cadd(v[1], v[2]);
         cplx<T> d = csub(v[1], v[2]);
         cplx<T> m = mk(v[0].x - T(0.5) * s.x, v[0].y - T(0.5) * s.y);
-        cplx<T> e = mul_si<T, SIGN>(mk(h * d.x, h * d.y));
+        cplx<T> e;
+        if constexpr (EX)
+            e = mul_si<T, SIGN>(mk(mulk(d.x, h), mulk(d.y, h)));
+        else
+            e = mul_si<T, SIGN>(mk((T)h * d.x, (T)h * d.y));
         v[0] = cadd(v[0], s);
         v[1] = cadd(m, e);
         v[2] = csub(m, e);
     }
 };
 
-template <typename T, int SIGN>
-struct Dft<T, 4, SIGN> {
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 4, SIGN, EX> {
     __device__ __forceinline__ static void run(cplx<T>* v) {
         cplx<T> t0 = cadd(v[0], v[2]);
         cplx<T> t1 = csub(v[0], v[2]);
@@ -117,7 +151,7 @@ struct Dft<T, 4, SIGN> {
 };
 
 // Composite DFT of length A*B in registers: n = B a + b, k = k1 + A k2.
-template <typename T, int A, int B, int SIGN>
+template <typename T, int A, int B, int SIGN, bool EX>
 __device__ __forceinline__ void dft_ab(cplx<T>* v) {
     cplx<T> y[A * B];
 #pragma unroll
@@ -125,41 +159,36 @@ __device__ __forceinline__ void dft_ab(cplx<T>* v) {
         cplx<T> t[A];
 #pragma unroll
         for (int a = 0; a < A; ++a) t[a] = v[B * a + b];
-        Dft<T, A, SIGN>::run(t);
+        Dft<T, A, SIGN, EX>::run(t);
 #pragma unroll
-        for (int k1 = 0; k1 < A; ++k1) {
-            if (b * k1 == 0)
-                y[b * A + k1] = t[k1];
-            else
-                y[b * A + k1] = cmul(t[k1], wconst<T, A * B, SIGN>(b * k1));
-        }
+        for (int k1 = 0; k1 < A; ++k1) y[b * A + k1] = mul_w<T, A * B, SIGN, EX>(t[k1], b * k1);
     }
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) {
         cplx<T> t[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) t[b] = y[b * A + k1];
-        Dft<T, B, SIGN>::run(t);
+        Dft<T, B, SIGN, EX>::run(t);
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = t[k2];
     }
 }
 
-template <typename T, int SIGN>
-struct Dft<T, 6, SIGN> {
-    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 2, 3, SIGN>(v); }
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 6, SIGN, EX> {
+    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 2, 3, SIGN, EX>(v); }
 };
-template <typename T, int SIGN>
-struct Dft<T, 8, SIGN> {
-    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 2, 4, SIGN>(v); }
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 8, SIGN, EX> {
+    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 2, 4, SIGN, EX>(v); }
 };
-template <typename T, int SIGN>
-struct Dft<T, 12, SIGN> {
-    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 4, 3, SIGN>(v); }
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 12, SIGN, EX> {
+    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 4, 3, SIGN, EX>(v); }
 };
-template <typename T, int SIGN>
-struct Dft<T, 16, SIGN> {
-    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 4, 4, SIGN>(v); }
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 16, SIGN, EX> {
+    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 4, 4, SIGN, EX>(v); }
 };
 
 constexpr int pick_radix(int L) {
@@ -190,8 +219,11 @@ constexpr bool supported_length(int L) {
 // distinct banks for every read and write.  (The butterfly-major order puts
 // the lanes' writes R*es apart, which is a multiple of the bank count for
 // the padded plane layouts: a 16-way conflict on the first step.)
-template <typename T, int L, int R, int NS, int NT, int MAXL, int SIGN>
-__device__ __forceinline__ void step(cplx<T>* buf, int nlines, int es, int ls, const cplx<T>* __restrict__ tw) {
+// EX: twiddles enter as the table value plus the lo table twl (tw + twl =
+// the FP64 twiddle to ~1e-15) and the register DFTs take the EX constants.
+template <typename T, int L, int R, int NS, int NT, int MAXL, int SIGN, bool EX>
+__device__ __forceinline__ void step(cplx<T>* buf, int nlines, int es, int ls, const cplx<T>* __restrict__ tw,
+                                     const cplx<T>* __restrict__ twl) {
     constexpr int NB = L / R;
     constexpr int MAXQ = (NB * MAXL + NT - 1) / NT;
     constexpr int TSTR = L / (NS * R);
@@ -220,10 +252,15 @@ __device__ __forceinline__ void step(cplx<T>* buf, int nlines, int es, int ls, c
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
                     const cplx<T> t = tw[r * k * TSTR];
-                    v[q][r] = (SIGN < 0) ? cmul(v[q][r], t) : cmulc(v[q][r], t);
+                    if constexpr (EX) {
+                        const cplx<T> tl = twl[r * k * TSTR];
+                        v[q][r] = (SIGN < 0) ? cmul2(v[q][r], t, tl) : cmulc2(v[q][r], t, tl);
+                    } else {
+                        v[q][r] = (SIGN < 0) ? cmul(v[q][r], t) : cmulc(v[q][r], t);
+                    }
                 }
             }
-            Dft<T, R, SIGN>::run(v[q]);
+            Dft<T, R, SIGN, EX>::run(v[q]);
             cplx<T>* base = buf + line * ls + ((j / NS) * NS * R + k) * es;
 #pragma unroll
             for (int r = 0; r < R; ++r) base[r * NS * es] = v[q][r];
@@ -232,23 +269,27 @@ __device__ __forceinline__ void step(cplx<T>* buf, int nlines, int es, int ls, c
     __syncthreads();
 }
 
-template <typename T, int L, int NS, int NT, int MAXL, int SIGN>
-__device__ __forceinline__ void run_steps(cplx<T>* buf, int nlines, int es, int ls, const cplx<T>* __restrict__ tw) {
+template <typename T, int L, int NS, int NT, int MAXL, int SIGN, bool EX>
+__device__ __forceinline__ void run_steps(cplx<T>* buf, int nlines, int es, int ls, const cplx<T>* __restrict__ tw,
+                                          const cplx<T>* __restrict__ twl) {
     if constexpr (NS < L) {
         constexpr int R = pick_radix(L / NS);
         static_assert(R > 1, "unsupported FFT length");
-        step<T, L, R, NS, NT, MAXL, SIGN>(buf, nlines, es, ls, tw);
-        run_steps<T, L, NS * R, NT, MAXL, SIGN>(buf, nlines, es, ls, tw);
+        step<T, L, R, NS, NT, MAXL, SIGN, EX>(buf, nlines, es, ls, tw, twl);
+        run_steps<T, L, NS * R, NT, MAXL, SIGN, EX>(buf, nlines, es, ls, tw, twl);
     }
 }
 
 // Unnormalised transform of `nlines` (<= MAXL) lines of length L held in LDS.
-// tw: the table W_L^m (m in [0, L)).  Callers place a barrier between the
+// tw: the table W_L^m (m in [0, L)); with EX (FP32 transforms of psi) twl
+// holds the lo parts W_L^m - tw[m].  Callers place a barrier between the
 // writes that filled the tile and this call.
-template <typename T, int L, int NT, int MAXL, int SIGN>
-__device__ __forceinline__ void fft(cplx<T>* buf, int nlines, int es, int ls, const cplx<T>* __restrict__ tw) {
+template <typename T, int L, int NT, int MAXL, int SIGN, bool EX = false>
+__device__ __forceinline__ void fft(cplx<T>* buf, int nlines, int es, int ls, const cplx<T>* __restrict__ tw,
+                                    const cplx<T>* __restrict__ twl = nullptr) {
     static_assert(supported_length(L), "FFT length must be 2^a 3^b");
-    run_steps<T, L, 1, NT, MAXL, SIGN>(buf, nlines, es, ls, tw);
+    constexpr bool X = EX && sizeof(T) == 4;
+    run_steps<T, L, 1, NT, MAXL, SIGN, X>(buf, nlines, es, ls, tw, twl);
 }
 
 }  // namespace ct

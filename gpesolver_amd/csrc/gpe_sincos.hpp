@@ -1,4 +1,5 @@
-// gpe_sincos.hpp -- sin and cos of an FP64 phase for the FP32 bulk path.
+// gpe_sincos.hpp -- the real-time phase factor exp(i theta) of an FP64
+// phase, applied to an FP32 or FP64 value.
 //
 // The phase theta is reduced in FP64 by the nearest multiple of pi/2,
 // r = theta - q pi/2 with |r| <= pi/4, and sin r, cos r are even / odd
@@ -9,9 +10,11 @@
 // of [-pi/4, pi/4] (relative error weighting), rounded to FP32; the maximum
 // absolute error of the FP32 evaluation is 4.3e-8 (sin) and 6.8e-8 (cos),
 // about one FP32 ulp (checked on a dense grid by tests/test_sincos_poly.py).
-// The quadrant q then rotates (sin, cos) by q quarter turns.
+// The quadrant q turns the value exactly by q quarter turns.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "gpe_fft.hpp"
 
 namespace gpe {
 
@@ -21,8 +24,12 @@ struct SinCosPoly {
     static constexpr float C[4] = {-0.5f, 0.04166661947965622f, -0.0013886665692552924f, 2.4382301489822567e-05f};
 };
 
-// (sin theta, cos theta) in FP32 for an FP64 theta of any magnitude.
-__device__ __forceinline__ void sincos_phase(double theta, float* s, float* c) {
+// c * exp(i theta) in FP32: c turned exactly by the quarter turns q, then
+// b + b (cos r - 1 + i sin r) with the increment formed first and one
+// rounding of the sum.  cos r - 1 = u * pc keeps its relative precision at
+// small r, where a rounded cos r would be 1 and |(cos r, sin r)| > 1 (a norm
+// gain of r^2 per multiply on every point with a small phase).
+__device__ __forceinline__ cplx<float> phase_rot(cplx<float> c, double theta) {
     const double q = rint(theta * 0.63661977236758134308);  // 2/pi
     const float r = (float)(theta - q * 1.57079632679489661923);
     const float u = r * r;
@@ -35,17 +42,28 @@ __device__ __forceinline__ void sincos_phase(double theta, float* s, float* c) {
     pc = pc * u + SinCosPoly::C[1];
     pc = pc * u + SinCosPoly::C[0];
     const float sr = r + r * u * ps;
-    const float cr = 1.0f + u * pc;
+    const float cm1 = u * pc;
     const int k = ((int)(long long)q) & 3;
-    // sin(r + k pi/2), cos(r + k pi/2)
-    *s = (k == 0) ? sr : (k == 1) ? cr : (k == 2) ? -sr : -cr;
-    *c = (k == 0) ? cr : (k == 1) ? -sr : (k == 2) ? -cr : sr;
+    // b = c * i^k
+    const float bx = (k == 0) ? c.x : (k == 1) ? -c.y : (k == 2) ? -c.x : c.y;
+    const float by = (k == 0) ? c.y : (k == 1) ? c.x : (k == 2) ? -c.y : -c.x;
+    const float dx = fmaf(bx, cm1, -(by * sr));
+    const float dy = fmaf(by, cm1, bx * sr);
+    cplx<float> o;
+    o.x = bx + dx;
+    o.y = by + dy;
+    return o;
 }
 
-// FP64 phase for the FP64 path: reduced to [-pi, pi], then the FP64 sincos.
-__device__ __forceinline__ void sincos_phase(double theta, double* s, double* c) {
+// FP64: the FP64 sincos of the reduced phase.
+__device__ __forceinline__ cplx<double> phase_rot(cplx<double> c, double theta) {
+    double s, co;
     theta -= 6.283185307179586477 * rint(theta * 0.15915494309189533577);
-    sincos(theta, s, c);
+    sincos(theta, &s, &co);
+    cplx<double> o;
+    o.x = c.x * co - c.y * s;
+    o.y = c.x * s + c.y * co;
+    return o;
 }
 
 }  // namespace gpe

@@ -124,8 +124,10 @@ struct PlaneArgs {
     double* part_out;
     const cplx<T>* tw;
     int tw_n;           // entries of the twiddle table (staged in LDS by the compiled shapes)
-    const cplx<T>* ez;  // exp(i fz(k)) / nz
-    const cplx<T>* ey;  // exp(i fy(j)) / ny
+    const cplx<T>* ez;  // (cos fz(k) - 1, sin fz(k)) of the z kinetic phase fz
+    const cplx<T>* ey;  // (cos fy(j) - 1, sin fy(j))
+    const cplx<T>* twly;  // lo parts of W_ny^m (FP32 transforms of psi)
+    const cplx<T>* twlz;  // lo parts of W_nz^m
     LinePlan pz2, py2, pz, py;
     double pinv_scale;  // (N_atoms / dV) / (8 N)
     double cdd;
@@ -152,7 +154,8 @@ struct LineArgs {
     cplx<T>* s1;
     cplx<T>* tk;
     const cplx<T>* tw;
-    const cplx<T>* ex;  // exp(i fx(i)) / nx
+    const cplx<T>* ex;  // (cos fx(i) - 1, sin fx(i))
+    const cplx<T>* twlx;  // lo parts of W_nx^m (FP32 transforms of psi)
     const double* kx2;  // padded lattice kx^2, [2nx]
     const double* ky2;  // padded lattice ky^2, [2ny]
     const double* kz2;  // padded lattice kz^2, [nz+1]
@@ -230,11 +233,13 @@ struct PlanePolGeneric {
         gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz2, nl, es, ls, S, tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw,
+                                 const cplx<T>*) {
         gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py, nl, es, ls, S, tw);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
+    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw,
+                                 const cplx<T>*) {
         gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz, nl, es, ls, S, tw);
     }
 };
@@ -268,13 +273,16 @@ struct PlanePolCT {
     __device__ static void fft_z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
         gpe::ct::fft<T, 2 * NZ, NT, NY, S>(b, nl, es, ls, tw + a.pz2.tw);
     }
+    // transforms of psi: twiddles with their lo parts (twl), EX constants
     template <typename T, int NT, int S>
-    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
-        gpe::ct::fft<T, NY, NT, NZ, S>(b, nl, es, ls, tw + a.py.tw);
+    __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw,
+                                 const cplx<T>* twl) {
+        gpe::ct::fft<T, NY, NT, NZ, S, true>(b, nl, es, ls, tw + a.py.tw, twl);
     }
     template <typename T, int NT, int S>
-    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw) {
-        gpe::ct::fft<T, NZ, NT, NY, S>(b, nl, es, ls, tw + a.pz.tw);
+    __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw,
+                                 const cplx<T>* twl) {
+        gpe::ct::fft<T, NZ, NT, NY, S, true>(b, nl, es, ls, tw + a.pz.tw, twl);
     }
 };
 
@@ -301,7 +309,7 @@ struct LinePolCT {
     }
     template <typename T, int NT, int S>
     __device__ static void fft_x(cplx<T>* b, int nl, int es, int ls, const LineArgs<T>& a) {
-        gpe::ct::fft<T, NX, NT, kLineMaxE / NX, S>(b, nl, es, ls, a.tw + a.px.tw);
+        gpe::ct::fft<T, NX, NT, kLineMaxE / NX, S, true>(b, nl, es, ls, a.tw + a.px.tw, a.twlx);
     }
 };
 
@@ -328,20 +336,21 @@ __device__ __forceinline__ cplx<T> shfl_xor1(cplx<T> v) {
 
 // z: rows Q[j][0..NZ) (row stride qs), j < ny.  NZ <= 16: one thread per
 // row.  NZ = 2A (A = 16 or 12): two lanes per row, lane h holding
-// x[2a + h]; DFT_A over a, twiddle W_NZ^(h k1), and the radix-2 stage across
-// the lane pair through shuffles (thread h then holds X[k1 + A h]); the
-// inverse runs the same steps backwards.  twz = W_NZ^m, ez = phase / NZ.
+// x[2a + h]; DFT_A over a, twiddle W_NZ^(h k1) (compile-time constants),
+// and the radix-2 stage across the lane pair through shuffles (thread h then
+// holds X[k1 + A h]); the inverse runs the same steps backwards.
+// ez = (cos f - 1, sin f) of the z kinetic phase f.
 template <typename T, int NZ, int NT>
-__device__ __forceinline__ void kin_z_reg(cplx<T>* Q, int ny, int qs, const cplx<T>* ez, const cplx<T>* twz) {
+__device__ __forceinline__ void kin_z_reg(cplx<T>* Q, int ny, int qs, const cplx<T>* ez) {
     if constexpr (NZ <= 16) {
         for (int j = threadIdx.x; j < ny; j += NT) {
             cplx<T> v[NZ];
 #pragma unroll
             for (int k = 0; k < NZ; ++k) v[k] = Q[j * qs + k];
-            gpe::ct::Dft<T, NZ, -1>::run(v);
+            gpe::ct::Dft<T, NZ, -1, true>::run(v);
 #pragma unroll
-            for (int k = 0; k < NZ; ++k) v[k] = gpe::cmul(v[k], ez[k]);
-            gpe::ct::Dft<T, NZ, +1>::run(v);
+            for (int k = 0; k < NZ; ++k) v[k] = gpe::kin_mul(v[k], ez[k], 1.0 / NZ);
+            gpe::ct::Dft<T, NZ, +1, true>::run(v);
 #pragma unroll
             for (int k = 0; k < NZ; ++k) Q[j * qs + k] = v[k];
         }
@@ -357,16 +366,16 @@ __device__ __forceinline__ void kin_z_reg(cplx<T>* Q, int ny, int qs, const cplx
             cplx<T> v[A];
 #pragma unroll
             for (int q = 0; q < A; ++q) v[q] = act ? Q[j * qs + 2 * q + h] : gpe::mk(T(0), T(0));
-            gpe::ct::Dft<T, A, -1>::run(v);
+            gpe::ct::Dft<T, A, -1, true>::run(v);
             if (h) {
 #pragma unroll
-                for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmul(v[k1], twz[k1]);
+                for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::ct::mul_w<T, NZ, -1, true>(v[k1], k1);
             }
 #pragma unroll
             for (int k1 = 0; k1 < A; ++k1) {
                 const cplx<T> o = shfl_xor1(v[k1]);
                 v[k1] = h ? gpe::csub(o, v[k1]) : gpe::cadd(v[k1], o);
-                v[k1] = gpe::cmul(v[k1], ez[k1 + A * h]);
+                v[k1] = gpe::kin_mul(v[k1], ez[k1 + A * h], 1.0 / NZ);
             }
 #pragma unroll
             for (int k1 = 0; k1 < A; ++k1) {
@@ -375,9 +384,9 @@ __device__ __forceinline__ void kin_z_reg(cplx<T>* Q, int ny, int qs, const cplx
             }
             if (h) {
 #pragma unroll
-                for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmulc(v[k1], twz[k1]);
+                for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::ct::mul_w<T, NZ, +1, true>(v[k1], k1);
             }
-            gpe::ct::Dft<T, A, +1>::run(v);
+            gpe::ct::Dft<T, A, +1, true>::run(v);
             if (act) {
 #pragma unroll
                 for (int q = 0; q < A; ++q) Q[j * qs + 2 * q + h] = v[q];
@@ -386,16 +395,26 @@ __device__ __forceinline__ void kin_z_reg(cplx<T>* Q, int ny, int qs, const cplx
     }
 }
 
+// a * tw[m] (SIGN -1) or a * conj(tw[m]) (SIGN +1) in a transform of psi:
+// in FP32 with the lo part twl[m], one rounding of the exact product.
+template <typename T, int SIGN>
+__device__ __forceinline__ cplx<T> tw_psi(cplx<T> a, const cplx<T>* tw, const cplx<T>* twl, int m) {
+    if constexpr (sizeof(T) == 4)
+        return SIGN < 0 ? gpe::cmul2(a, tw[m], twl[m]) : gpe::cmulc2(a, tw[m], twl[m]);
+    else
+        return SIGN < 0 ? gpe::cmul(a, tw[m]) : gpe::cmulc(a, tw[m]);
+}
+
 // y: columns c < NZ of Q[j][c] (row stride qs), length NY = 16 B (B = 4 or
 // 8), four-step with one LDS exchange each way: thread (b, c) holds rows
 // j = B a + b; DFT_16 over a, twiddle W_NY^(b k1), exchange, DFT_B over b for
-// its rows k1 = b + B r, x phase ey, inverse DFT_B, conjugate twiddle,
+// its rows k1 = b + B r, y phase ey, inverse DFT_B, conjugate twiddle,
 // exchange back, inverse DFT_16; the result goes straight from registers
 // to Tk[j][i][c] (global).  The exchange area aliases Q.  Every thread of the
 // block reaches the barriers.
 template <typename T, int NY, int NZ, int NT>
 __device__ __forceinline__ void kin_y_reg_store(cplx<T>* Q, int qs, const cplx<T>* ey, const cplx<T>* twy,
-                                                cplx<T>* tk_plane, int nx) {
+                                                const cplx<T>* twyl, cplx<T>* tk_plane, int nx) {
     constexpr int A = 16, B = NY / A, R = A / B;
     static_assert(NY == A * B && A % B == 0 && B * NZ <= NT, "y length of the register pass");
     constexpr int ES = NZ + 1;  // exchange row stride (odd: distinct banks)
@@ -405,9 +424,9 @@ __device__ __forceinline__ void kin_y_reg_store(cplx<T>* Q, int qs, const cplx<T
     if (act) {
 #pragma unroll
         for (int q = 0; q < A; ++q) v[q] = Q[(B * q + b) * qs + c];
-        gpe::ct::Dft<T, A, -1>::run(v);
+        gpe::ct::Dft<T, A, -1, true>::run(v);
 #pragma unroll
-        for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmul(v[k1], twy[(b * k1) % NY]);
+        for (int k1 = 1; k1 < A; ++k1) v[k1] = tw_psi<T, -1>(v[k1], twy, twyl, (b * k1) % NY);
     }
     __syncthreads();  // Q is read: the exchange area may overwrite it
     cplx<T>* E = Q;
@@ -423,12 +442,12 @@ __device__ __forceinline__ void kin_y_reg_store(cplx<T>* Q, int qs, const cplx<T
             const int k1 = b + B * r;
 #pragma unroll
             for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * ES + c];
-            gpe::ct::Dft<T, B, -1>::run(w[r]);
+            gpe::ct::Dft<T, B, -1, true>::run(w[r]);
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::cmul(w[r][k2], ey[k1 + A * k2]);
-            gpe::ct::Dft<T, B, +1>::run(w[r]);
+            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::kin_mul(w[r][k2], ey[k1 + A * k2], 1.0 / NY);
+            gpe::ct::Dft<T, B, +1, true>::run(w[r]);
 #pragma unroll
-            for (int bb = 1; bb < B; ++bb) w[r][bb] = gpe::cmulc(w[r][bb], twy[(bb * k1) % NY]);
+            for (int bb = 1; bb < B; ++bb) w[r][bb] = tw_psi<T, +1>(w[r][bb], twy, twyl, (bb * k1) % NY);
         }
     }
     __syncthreads();
@@ -444,7 +463,7 @@ __device__ __forceinline__ void kin_y_reg_store(cplx<T>* Q, int qs, const cplx<T
     if (act) {
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * ES + c];
-        gpe::ct::Dft<T, A, +1>::run(v);
+        gpe::ct::Dft<T, A, +1, true>::run(v);
 #pragma unroll
         for (int q = 0; q < A; ++q) tk_plane[(size_t)(B * q + b) * nx * NZ + c] = v[q];
     }
@@ -564,7 +583,8 @@ __device__ __forceinline__ void y2_fwd_reg_store(cplx<T>* P, int zs, const cplx<
 // ---------------------------------------------------------------------------
 // Plane kernel.  Dynamic LDS: the plane tile, max(2ny*(nz+1), ny*(2nz+1))
 // complex T, followed for the compiled shapes by the staged tables: the
-// twiddle table (tw_n entries), ez (nz) and ey (ny).
+// twiddle table (tw_n entries), ez (nz), ey (ny) and the lo parts of the
+// W_ny (ny) and W_nz (nz) twiddles.
 template <typename T, int NT, class Pol>
 __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, const int mode, unsigned char* lds_raw,
                                            double* red) {
@@ -592,6 +612,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
     const cplx<T>* tw = a.tw;
     const cplx<T>* ez = a.ez;
     const cplx<T>* ey = a.ey;
+    const cplx<T>* twly = a.twly;
+    const cplx<T>* twlz = a.twlz;
     constexpr bool kRegY2 = Pol::kRegY2;
     constexpr int R2 = (kStage && !kRegY2) ? Pol::template rq2<NT>() : 1;
     cplx<T> st[R2];        // S1 plane (Poisson finish input), element order
@@ -638,10 +660,16 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         if (mode & M_KIN) {
             cplx<T>* EZ = TW + a.tw_n;
             cplx<T>* EY = EZ + nz;
+            cplx<T>* TWLY = EY + ny;
+            cplx<T>* TWLZ = TWLY + ny;
             for (int m = threadIdx.x; m < nz; m += NT) EZ[m] = a.ez[m];
             for (int m = threadIdx.x; m < ny; m += NT) EY[m] = a.ey[m];
+            for (int m = threadIdx.x; m < ny; m += NT) TWLY[m] = a.twly[m];
+            for (int m = threadIdx.x; m < nz; m += NT) TWLZ[m] = a.twlz[m];
             ez = EZ;
             ey = EY;
+            twly = TWLY;
+            twlz = TWLZ;
         }
     }
 
@@ -884,9 +912,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                         const double dn = (double)nrm;
                         const double vv = kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
                         const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
-                        T sn, cs;
-                        gpe::sincos_phase(-a.hdt * veff, &sn, &cs);
-                        c = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
+                        c = gpe::phase_rot(c, -a.hdt * veff);
                         Q[qi] = c;
                         nrm = c.x * c.x + c.y * c.y;
                     }
@@ -905,16 +931,16 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
             if constexpr (Pol::kRegZ) {
                 // tables named through the LDS base, so their reads compile to LDS loads
                 const cplx<T>* TWs = P + max(ny2 * nzh, ny * zs);
-                kin_z_reg<T, Pol::kNZ, NT>(Q, ny, qs, TWs + a.tw_n, TWs + a.pz.tw);
+                kin_z_reg<T, Pol::kNZ, NT>(Q, ny, qs, TWs + a.tw_n);
                 __syncthreads();
             } else {
-                Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a, tw);
+                Pol::template fft_z<T, NT, -1>(Q, ny, 1, qs, a, tw, twlz);
                 for (int p = threadIdx.x; p < npl; p += NT) {
                     const int qi = (p / nz) * qs + p % nz;
-                    Q[qi] = gpe::cmul(Q[qi], ez[p % nz]);
+                    Q[qi] = gpe::kin_mul(Q[qi], ez[p % nz], 1.0 / nz);
                 }
                 __syncthreads();
-                Pol::template fft_z<T, NT, +1>(Q, ny, 1, qs, a, tw);
+                Pol::template fft_z<T, NT, +1>(Q, ny, 1, qs, a, tw, twlz);
             }
         }
         if ((mode & M_KIN) && Pol::kRegY) {
@@ -922,17 +948,17 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
             if constexpr (Pol::kRegY) {
                 const cplx<T>* TWs = P + max(ny2 * nzh, ny * zs);
                 kin_y_reg_store<T, Pol::kNY, Pol::kNZ, NT>(Q, qs, TWs + a.tw_n + nz, TWs + a.py.tw,
-                                                           a.tko + (size_t)i * nz, nx);
+                                                           TWs + a.tw_n + nz + ny, a.tko + (size_t)i * nz, nx);
             }
         } else if (mode & M_KIN) {
             // y propagation: columns k, length ny
-            Pol::template fft_y<T, NT, -1>(Q, nz, qs, 1, a, tw);
+            Pol::template fft_y<T, NT, -1>(Q, nz, qs, 1, a, tw, twly);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int qi = (p / nz) * qs + p % nz;
-                Q[qi] = gpe::cmul(Q[qi], ey[p / nz]);
+                Q[qi] = gpe::kin_mul(Q[qi], ey[p / nz], 1.0 / ny);
             }
             __syncthreads();
-            Pol::template fft_y<T, NT, +1>(Q, nz, qs, 1, a, tw);
+            Pol::template fft_y<T, NT, +1>(Q, nz, qs, 1, a, tw, twly);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
                 a.tko[((size_t)j * nx + i) * nz + k] = Q[j * qs + k];
@@ -1072,7 +1098,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
         Pol::template fft_x<T, NT, -1>(X, C, CS, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            X[ii * CS + c] = gpe::cmul(X[ii * CS + c], a.ex[ii]);
+            X[ii * CS + c] = gpe::kin_mul(X[ii * CS + c], a.ex[ii], 1.0 / nx);
         }
         __syncthreads();
         Pol::template fft_x<T, NT, +1>(X, C, CS, 1, a);
@@ -1147,7 +1173,7 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
     }
     // issue the line loads first, then stage the tables while they are in
     // flight: TB[0, L) = W_L^m, then (Poisson) kx^2 as T or (kinetic) the
-    // x phase table
+    // x phase table and the lo parts of W_L^m
     constexpr int AL = POISSON ? A / 2 : A;  // Poisson: x >= nx is the zero padding
     cplx<T> v[A];
 #pragma unroll
@@ -1163,18 +1189,27 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
     const cplx<T>* tw = TB;
     T* kx2s = reinterpret_cast<T*>(TB + L);
     const cplx<T>* exs = TB + L;
+    const cplx<T>* twl = TB + 2 * L;
     for (int m = threadIdx.x; m < L; m += 256) {
         TB[m] = a.tw[(POISSON ? a.px2.tw : a.px.tw) + m];
-        if (POISSON)
+        if (POISSON) {
             kx2s[m] = (T)a.kx2[m];
-        else
+        } else {
             TB[L + m] = a.ex[m];
+            TB[2 * L + m] = a.twlx[m];
+        }
     }
     __syncthreads();
-    // forward, stage 1
-    gpe::ct::Dft<T, A, -1>::run(v);
+    // forward, stage 1 (kinetic lines are transforms of psi: EX constants and
+    // twiddles with their lo parts)
+    gpe::ct::Dft<T, A, -1, !POISSON>::run(v);
 #pragma unroll
-    for (int k1 = 1; k1 < A; ++k1) v[k1] = gpe::cmul(v[k1], tw[(b * k1) % L]);
+    for (int k1 = 1; k1 < A; ++k1) {
+        if constexpr (POISSON)
+            v[k1] = gpe::cmul(v[k1], tw[(b * k1) % L]);
+        else
+            v[k1] = tw_psi<T, -1>(v[k1], tw, twl, (b * k1) % L);
+    }
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = v[k1];
     __syncthreads();
@@ -1185,7 +1220,7 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
         const int k1 = b + B * r;
 #pragma unroll
         for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * LS + ll];
-        gpe::ct::Dft<T, B, -1>::run(w[r]);
+        gpe::ct::Dft<T, B, -1, !POISSON>::run(w[r]);
         if (POISSON) {
 #pragma unroll
             for (int k2 = 0; k2 < B; ++k2) {
@@ -1195,11 +1230,16 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
             }
         } else {
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::cmul(w[r][k2], exs[k1 + A * k2]);
+            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::kin_mul(w[r][k2], exs[k1 + A * k2], 1.0 / L);
         }
-        gpe::ct::Dft<T, B, +1>::run(w[r]);
+        gpe::ct::Dft<T, B, +1, !POISSON>::run(w[r]);
 #pragma unroll
-        for (int bb = 1; bb < B; ++bb) w[r][bb] = gpe::cmulc(w[r][bb], tw[(bb * k1) % L]);
+        for (int bb = 1; bb < B; ++bb) {
+            if constexpr (POISSON)
+                w[r][bb] = gpe::cmulc(w[r][bb], tw[(bb * k1) % L]);
+            else
+                w[r][bb] = tw_psi<T, +1>(w[r][bb], tw, twl, (bb * k1) % L);
+        }
     }
     __syncthreads();  // every stage-2 read of E is done before it is rewritten
 #pragma unroll
@@ -1212,7 +1252,7 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
     // inverse stage 2: thread b gathers Z[k1][b] for every k1
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * LS + ll];
-    gpe::ct::Dft<T, A, +1>::run(v);
+    gpe::ct::Dft<T, A, +1, !POISSON>::run(v);
     if (valid) {
         // streaming stores: the next pass reads them from memory anyway, and
         // lines that do not linger dirty in L2 shorten the end-of-kernel flush
@@ -1228,7 +1268,8 @@ struct XReg {
     static constexpr int LPB_K = 256 / (LK / 16);
 };
 
-// LDS: exchange area (16 * 256 + 256 complex) + tables (2 * 2NX complex).
+// LDS: exchange area (16 * 256 + 256 complex) + tables (Poisson 2 * 2NX,
+// kinetic 3 * NX complex).
 // The kinetic tiles (twice the loads of a zero-padded Poisson tile) take the
 // first block indices so that they are dispatched first.
 template <typename T, int NX>
@@ -1328,9 +1369,7 @@ __global__ void k_potential_half(cplx<T>* psi, const T* V, const T* phi, int nx,
         cplx<T> c = psi[e];
         const T nr = c.x * c.x + c.y * c.y;
         const double veff = (double)V[e] + cdd * (double)phi[e] + pg * (double)nr + pgm * rho32(nr);
-        T sn, cs;
-        gpe::sincos_phase(-hdt * veff, &sn, &cs);
-        psi[e] = gpe::mk(c.x * cs - c.y * sn, c.x * sn + c.y * cs);
+        psi[e] = gpe::phase_rot(c, -hdt * veff);
     }
 }
 
@@ -1466,6 +1505,7 @@ struct gpe_handle {
     void* s1b = nullptr;  // second S1 / Tk of the split real-time plane pass (allocated on first use)
     void* tkb = nullptr;
     void* tw = nullptr;
+    void* twl = nullptr;  // lo parts of the W_nx, W_ny, W_nz twiddles, [nx][ny][nz]
     void* ex = nullptr;
     void* ey = nullptr;
     void* ez = nullptr;
@@ -1579,14 +1619,49 @@ static int download_real(gpe_handle* h, double* dst, const void* src, size_t n, 
     return 0;
 }
 
+// Table entry for the complex constant c + i s.  FP64: rounded to nearest.
+// FP32: among the FP32 pairs within one unit in the last place of (c, s),
+// the one whose magnitude is closest to sqrt(c^2 + s^2), ties and near ties
+// settled by `wang` times the angle error.  Every table entry enters a
+// unitary product (a transform pass, a phase factor); nearest rounding
+// leaves |w| short of its exact value for most angles used (cos = sin at
+// pi/4, for instance), and the products then lose norm every step.
+template <typename T>
+static cplx<T> fit_unit(double c, double s, double wang) {
+    cplx<T> r;
+    r.x = (T)c;
+    r.y = (T)s;
+    if (sizeof(T) == 8) return r;
+    const double m2 = c * c + s * s, m = sqrt(m2);
+    double best = 1e300;
+    const float c0 = (float)c, s0 = (float)s;
+    const float cs[3] = {nextafterf(c0, -INFINITY), c0, nextafterf(c0, INFINITY)};
+    const float ss[3] = {nextafterf(s0, -INFINITY), s0, nextafterf(s0, INFINITY)};
+    for (float cc : cs)
+        for (float sv : ss) {
+            const double dm = fabs(((double)cc * cc + (double)sv * sv) - m2) / (2.0 * m2);
+            // angle error: component of the error orthogonal to (c, s), relative to |w|
+            const double da = fabs(((double)sv - s) * c - ((double)cc - c) * s) / m2;
+            const double cost = dm + wang * da;
+            if (cost < best) {
+                best = cost;
+                r.x = (T)cc;
+                r.y = (T)sv;
+            }
+        }
+    return r;
+}
+
 template <typename T>
 static void fill_phase(std::vector<cplx<T>>& out, int n, double d, double dt, double m) {
     out.resize(n);
     for (int t = 0; t < n; ++t) {
         const double k = (t <= n / 2) ? 2.0 * M_PI * t / (n * d) : 2.0 * M_PI * (t - n) / (n * d);
         const double f = -dt / (2.0 * m) * k * k;
-        out[t].x = (T)(cos(f) / n);
-        out[t].y = (T)(sin(f) / n);
+        // (cos f - 1, sin f); the kernels apply (1 + e) / n
+        const double sh = sin(0.5 * f);
+        out[t].x = (T)(-2.0 * sh * sh);
+        out[t].y = (T)sin(f);
     }
 }
 
@@ -1603,15 +1678,30 @@ static int setup_tables(gpe_handle* h) {
         pls[t]->tw = (int)tw.size();
         for (int m = 0; m < lens[t]; ++m) {
             const double ang = 2.0 * M_PI * m / lens[t];
-            cplx<T> c;
-            c.x = (T)cos(ang);
-            c.y = (T)(-sin(ang));
-            tw.push_back(c);
+            // a twiddle's angle error largely cancels between a forward pass
+            // and the inverse pass with the conjugate twiddle; its magnitude
+            // error does not
+            tw.push_back(fit_unit<T>(cos(ang), -sin(ang), 0.25));
         }
     }
     h->tw_n = (int)tw.size();
     GPE_CHECK(hipMalloc(&h->tw, tw.size() * sizeof(cplx<T>)));
     GPE_CHECK(hipMemcpy(h->tw, tw.data(), tw.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    // lo parts W_L^m - tw[m] of the psi transform lengths (FP32: the
+    // transforms of psi multiply by tw + twl, see tw_psi)
+    std::vector<cplx<T>> twl;
+    for (int t = 3; t < 6; ++t) {
+        for (int m = 0; m < lens[t]; ++m) {
+            const double ang = 2.0 * M_PI * m / lens[t];
+            const cplx<T> hi = tw[pls[t]->tw + m];
+            cplx<T> lo;
+            lo.x = (T)(cos(ang) - (double)hi.x);
+            lo.y = (T)(-sin(ang) - (double)hi.y);
+            twl.push_back(lo);
+        }
+    }
+    GPE_CHECK(hipMalloc(&h->twl, twl.size() * sizeof(cplx<T>)));
+    GPE_CHECK(hipMemcpy(h->twl, twl.data(), twl.size() * sizeof(cplx<T>), hipMemcpyHostToDevice));
     // kinetic phase tables (fftw_split_solver.cpp:11-42, factorised per axis)
     std::vector<cplx<T>> e;
     fill_phase<T>(e, p.nx, p.dx, p.real_time_dt, p.m);
@@ -1669,6 +1759,8 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.tw_n = h->tw_n;
     a.ez = (const cplx<T>*)h->ez;
     a.ey = (const cplx<T>*)h->ey;
+    a.twly = (const cplx<T>*)h->twl + p.nx;
+    a.twlz = (const cplx<T>*)h->twl + p.nx + p.ny;
     a.pz2 = h->pz2;
     a.py2 = h->py2;
     a.pz = h->pz;
@@ -1706,6 +1798,7 @@ static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
     a.tk = (cplx<T>*)h->tk;
     a.tw = (const cplx<T>*)h->tw;
     a.ex = (const cplx<T>*)h->ex;
+    a.twlx = (const cplx<T>*)h->twl;
     a.kx2 = h->kx2;
     a.ky2 = h->ky2;
     a.kz2 = h->kz2;
@@ -2449,7 +2542,7 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     h->ck = std::min(p->nz, std::max(1, kLineMaxE / p->nx));
     h->ck = std::min(h->ck, 16);
     h->plane_lds = (size_t)std::max(2L * p->ny * (p->nz + 1), (long)p->ny * (2 * p->nz + 1)) * cs;
-    h->plane_stage = (size_t)(h->tw_n + p->ny + p->nz) * cs;
+    h->plane_stage = (size_t)(h->tw_n + 2 * (p->ny + p->nz)) * cs;
     h->line_lds = (size_t)std::max(2L * p->nx * (h->cp + 1), (long)p->nx * (h->ck + 1)) * cs;
     if (hipDeviceSynchronize() != hipSuccess) return bad("device synchronize failed after setup");
     *out = h;
@@ -2460,7 +2553,7 @@ void gpe_destroy(gpe_handle* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->store[0], h->store[1], h->psii,     h->V,       h->phi,    h->s1,  h->tk,
-                    h->s1b,      h->tkb,      h->tw,       h->ex,      h->ey,     h->ez,  h->kx2,
+                    h->s1b,      h->tkb,      h->tw,       h->twl,     h->ex,     h->ey,  h->ez,  h->kx2,
                     h->ky2,      h->kz2,      h->part[0],  h->part[1], h->epart};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
