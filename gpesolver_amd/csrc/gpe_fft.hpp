@@ -183,12 +183,13 @@ __device__ __forceinline__ void dft_small(cplx<T> (&a)[R], int sign) {
 }
 
 // One Stockham step of radix R on `nlines` lines held in LDS.
-// Element e of line l lives at buf[l*ls + e*es].  tw points at W_L^m.
+// Element e of line l lives at buf[l*ls + e*es].  tw points at W_L^m; twl,
+// when given (FP32 transforms of psi), at the lo parts W_L^m - tw[m].
 // MAXE bounds the number of complex elements of the tile (compile time) so
 // that each thread's butterflies fit in registers.
 template <typename T, int NT, int MAXE, int R>
 __device__ __forceinline__ void stockham_step(cplx<T>* buf, int L, int Ns, int nlines, int es, int ls, int sign,
-                                              const cplx<T>* __restrict__ tw) {
+                                              const cplx<T>* __restrict__ tw, const cplx<T>* __restrict__ twl) {
     constexpr int MAXQ = (MAXE / R + NT - 1) / NT;
     const int nb = L / R;
     const int total = nb * nlines;
@@ -217,7 +218,12 @@ __device__ __forceinline__ void stockham_step(cplx<T>* buf, int L, int Ns, int n
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
                     cplx<T> t = tw[r * k * tstride];
-                    v[q][r] = (sign < 0) ? cmul(v[q][r], t) : cmulc(v[q][r], t);
+                    if (sizeof(T) == 4 && twl) {
+                        const cplx<T> tl = twl[r * k * tstride];
+                        v[q][r] = (sign < 0) ? cmul2(v[q][r], t, tl) : cmulc2(v[q][r], t, tl);
+                    } else {
+                        v[q][r] = (sign < 0) ? cmul(v[q][r], t) : cmulc(v[q][r], t);
+                    }
                 }
             }
             dft_small<T, R>(v[q], sign);
@@ -233,23 +239,24 @@ __device__ __forceinline__ void stockham_step(cplx<T>* buf, int L, int Ns, int n
 // Full transform of every line (unnormalised; sign -1 forward, +1 backward).
 // Requires a __syncthreads() between the writes that filled the tile and this
 // call (callers provide it).
+// twl (optional): the lo parts of this length's twiddles (transforms of psi).
 template <typename T, int NT, int MAXE>
 __device__ void lds_fft(cplx<T>* buf, const LinePlan& pl, int nlines, int es, int ls, int sign,
-                        const cplx<T>* __restrict__ twbase) {
+                        const cplx<T>* __restrict__ twbase, const cplx<T>* __restrict__ twl = nullptr) {
     const cplx<T>* tw = twbase + pl.tw;
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.radix[s];
         if (R == 4)
-            stockham_step<T, NT, MAXE, 4>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
+            stockham_step<T, NT, MAXE, 4>(buf, pl.L, Ns, nlines, es, ls, sign, tw, twl);
         else if (R == 2)
-            stockham_step<T, NT, MAXE, 2>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
+            stockham_step<T, NT, MAXE, 2>(buf, pl.L, Ns, nlines, es, ls, sign, tw, twl);
         else if (R == 3)
-            stockham_step<T, NT, MAXE, 3>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
+            stockham_step<T, NT, MAXE, 3>(buf, pl.L, Ns, nlines, es, ls, sign, tw, twl);
         else if (R == 5)
-            stockham_step<T, NT, MAXE, 5>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
+            stockham_step<T, NT, MAXE, 5>(buf, pl.L, Ns, nlines, es, ls, sign, tw, twl);
         else
-            stockham_step<T, NT, MAXE, 7>(buf, pl.L, Ns, nlines, es, ls, sign, tw);
+            stockham_step<T, NT, MAXE, 7>(buf, pl.L, Ns, nlines, es, ls, sign, tw, twl);
         Ns *= R;
     }
 }

@@ -140,6 +140,12 @@ struct PlaneArgs {
     const double* coef;
     const int* cbase;
     int cidx;
+    // pencil pass (planes beyond one block's LDS): per-plane intermediates in
+    // HBM, [x][y][c] like psi, and the tile shapes
+    cplx<T>* pen_gw;  // G (inverse y2 of S1), then W (z-propagated psi)
+    cplx<T>* pen_h;   // H (z R2C of rho, kz = 1..nz)
+    int pen_c;        // columns per column tile
+    int pen_r;        // rows per row tile
 };
 
 template <typename T>
@@ -234,13 +240,13 @@ struct PlanePolGeneric {
     }
     template <typename T, int NT, int S>
     __device__ static void fft_y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw,
-                                 const cplx<T>*) {
-        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py, nl, es, ls, S, tw);
+                                 const cplx<T>* twl) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.py, nl, es, ls, S, tw, twl);
     }
     template <typename T, int NT, int S>
     __device__ static void fft_z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a, const cplx<T>* tw,
-                                 const cplx<T>*) {
-        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz, nl, es, ls, S, tw);
+                                 const cplx<T>* twl) {
+        gpe::lds_fft<T, NT, Lim<T>::maxe>(b, a.pz, nl, es, ls, S, tw, twl);
     }
 };
 
@@ -295,7 +301,7 @@ struct LinePolGeneric {
     }
     template <typename T, int NT, int S>
     __device__ static void fft_x(cplx<T>* b, int nl, int es, int ls, const LineArgs<T>& a) {
-        gpe::lds_fft<T, NT, kLineMaxE>(b, a.px, nl, es, ls, S, a.tw);
+        gpe::lds_fft<T, NT, kLineMaxE>(b, a.px, nl, es, ls, S, a.tw, a.twlx);
     }
 };
 
@@ -1049,6 +1055,288 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
 }
 
 // ---------------------------------------------------------------------------
+// Pencil plane pass: planes whose Poisson tile (2ny x (nz+1) complex) does
+// not fit one block's LDS.  One block per x-plane runs the modes of
+// plane_body in three sweeps over LDS tiles, with the plane-sized
+// intermediates in HBM (written and read back by the same block; most of it
+// stays in L2):
+//   (A) column tiles: inverse y2 of S1 -> G[y][kz-1], y < ny
+//   (B) row tiles: z C2R of G -> Phi; psi update (IT stencil or RT phase);
+//       z R2C of rho -> H[y][kz-1]; z kinetic of psi -> W (over G's rows)
+//   (C) column tiles: y2 of H -> S1o; y kinetic of W -> Tk
+// Same arithmetic as plane_body's run-time-size path.
+constexpr int kPencilThreads = 512;
+constexpr int kPencilPPT = 4;  // row-tile points per thread
+
+template <typename T>
+struct PencilLim;  // complex elements of one tile
+template <>
+struct PencilLim<float> {
+    static constexpr int maxe = 8192;
+};
+template <>
+struct PencilLim<double> {
+    static constexpr int maxe = 4096;
+};
+
+template <typename T, int NT>
+__device__ __forceinline__ void pencil_body(const PlaneArgs<T>& a, const int i, const int mode,
+                                            unsigned char* lds_raw, double* red) {
+    constexpr int MAXE = PencilLim<T>::maxe;
+    constexpr int PPT = kPencilPPT;
+    cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int ny2 = 2 * ny, nz2 = 2 * nz, npl = ny * nz;
+    const size_t plane_off = (size_t)i * npl;
+    const bool is_it = (mode & M_IT) != 0;
+    const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
+    const int C = a.pen_c, R = a.pen_r;
+    cplx<T>* gw = a.pen_gw + plane_off;  // [y][c]
+    cplx<T>* hs = a.pen_h + plane_off;   // [y][kz - 1]
+
+    double pg = a.pg, pgm = a.pgm;
+    if (a.coef && (mode & (M_PHASE_HALF | M_PHASE_FULL))) {
+        const int t = (a.cbase ? *a.cbase : 0) + a.cidx;
+        pg = a.coef[2 * t];
+        pgm = a.coef[2 * t + 1];
+        if (mode & M_PHASE_FULL) {
+            pg += a.coef[2 * t + 2];
+            pgm += a.coef[2 * t + 3];
+        }
+    }
+    double s_in = 1.0;
+    if (mode & M_NORM_IN) s_in = 1.0 / sqrt(sum_partials(a.part_in, nx, red));
+
+    // ---- (A) inverse y2 of the S1 columns (kz = 1..nz) -> G
+    if (mode & M_PINV) {
+        const int cs = C + 1;
+        for (int c0 = 0; c0 < nz; c0 += C) {
+            const int cw = min(C, nz - c0);
+            for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
+                const int ky = e / cw, c = e - ky * cw;
+                X[ky * cs + c] = a.s1[((size_t)ky * nx + i) * nz + c0 + c];
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.py2, cw, cs, 1, +1, a.tw);
+            for (int e = threadIdx.x; e < ny * cw; e += NT) {
+                const int y = e / cw, c = e - y * cw;
+                gw[(size_t)y * nz + c0 + c] = X[y * cs + c];
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- (B) row tiles
+    double nacc = 0.0;
+    const int zs = nz2 + 1, qs = nz + 1;
+    const bool iint = (i >= 1 && i <= nx - 2);
+    for (int y0 = 0; y0 < ny; y0 += R) {
+        const int rw = min(R, ny - y0);
+        const int np = rw * nz;
+        T phi_r[PPT], rho_r[PPT];
+        cplx<T> cv[PPT];
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            phi_r[q] = T(0);
+            rho_r[q] = T(0);
+            cv[q] = gpe::mk(T(0), T(0));
+        }
+        // Phi of the tile rows: z C2R of G (Hermitian extension, kz = 0 zero)
+        if (mode & M_PINV) {
+            for (int e = threadIdx.x; e < rw * (nz + 1); e += NT) {
+                const int r = e / (nz + 1), kz = e - r * (nz + 1);
+                const cplx<T> v = kz ? gw[(size_t)(y0 + r) * nz + kz - 1] : gpe::mk(T(0), T(0));
+                X[r * zs + kz] = v;
+                if (kz >= 1 && kz < nz) X[r * zs + nz2 - kz] = gpe::mk(v.x, -v.y);
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.pz2, rw, 1, zs, +1, a.tw);
+            const T sc = (T)(a.pinv_scale * s_in * s_in);
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < np) {
+                    const int r = p / nz, z = p - r * nz;
+                    phi_r[q] = X[r * zs + z].x * sc;
+                    if (mode & M_PHI_STORE) a.phi[plane_off + (size_t)(y0 + r) * nz + z] = phi_r[q];
+                }
+            }
+            __syncthreads();
+        } else if (mode & M_PHI_LOAD) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < np) phi_r[q] = a.phi[plane_off + (size_t)y0 * nz + p];
+            }
+        }
+        // psi update
+        if (is_it) {
+            const cplx<T>* in = a.in;
+            const T sv = (T)s_in;
+            const T dtau = (T)a.dtau, cx = (T)a.cx, cy = (T)a.cy, cz = (T)a.cz, gq = (T)a.gq, gmq = (T)a.gmq,
+                    cdd = (T)a.cdd;
+            auto ld = [&](size_t e) {
+                const cplx<T> v = in[e];
+                return gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+            };
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < np) {
+                    const int r = p / nz, k = p - r * nz, j = y0 + r;
+                    const size_t e = plane_off + (size_t)j * nz + k;
+                    cplx<T> o;
+                    if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
+                        const cplx<T> c = ld(e);
+                        const cplx<T> xm = in[e - npl], xp = in[e + npl];
+                        const cplx<T> ym = ld(e - nz), yp = ld(e + nz), zm = ld(e - 1), zp = ld(e + 1);
+                        const T v = a.V[e] + cdd * phi_r[q];
+                        cplx<T> c1;
+                        c1.x = cx * (xm.x * sv + xp.x * sv - T(2) * c.x) + cy * (ym.x + yp.x - T(2) * c.x) +
+                               cz * (zm.x + zp.x - T(2) * c.x) + c.x * v;
+                        c1.y = cx * (xm.y * sv + xp.y * sv - T(2) * c.y) + cy * (ym.y + yp.y - T(2) * c.y) +
+                               cz * (zm.y + zp.y - T(2) * c.y) + c.y * v;
+                        const T nrm = c.x * c.x + c.y * c.y;
+                        const T ab = sqrt(nrm);
+                        const T f = gq * nrm + gmq * (ab * ab * ab);
+                        o.x = c.x - dtau * c1.x;
+                        o.y = c.y - dtau * c1.y;
+                        o.x = o.x - dtau * (f * c.x);
+                        o.y = o.y - dtau * (f * c.y);
+                    } else {
+                        o = a.psii[e];
+                    }
+                    a.out[e] = o;
+                    cv[q] = o;
+                    rho_r[q] = o.x * o.x + o.y * o.y;
+                    nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+                }
+            }
+        } else if (have_psi) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < np) {
+                    const int r = p / nz, k = p - r * nz, j = y0 + r;
+                    const size_t e = plane_off + (size_t)j * nz + k;
+                    cplx<T> c;
+                    if (mode & M_IN_TK) {
+                        c = a.tk[((size_t)j * nx + i) * nz + k];
+                    } else {
+                        c = a.in[e];
+                        if (mode & M_NORM_IN) c = gpe::mk((T)(c.x * s_in), (T)(c.y * s_in));
+                    }
+                    T nrm = c.x * c.x + c.y * c.y;
+                    if ((mode & (M_PHASE_HALF | M_PHASE_FULL)) && iint && j >= 1 && j <= ny - 2 && k >= 1 &&
+                        k <= nz - 2) {
+                        const double dn = (double)nrm;
+                        const double veff =
+                            a.cv * ((double)a.V[e] + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
+                        c = gpe::phase_rot(c, -a.hdt * veff);
+                        nrm = c.x * c.x + c.y * c.y;
+                    }
+                    cv[q] = c;
+                    rho_r[q] = nrm;
+                    nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
+                    if (mode & M_OUT) a.out[e] = c;
+                }
+            }
+        }
+        // z R2C of rho -> H
+        if (mode & M_RHO_FWD) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < np) {
+                    const int r = p / nz, z = p - r * nz;
+                    X[r * zs + z] = gpe::mk(rho_r[q], T(0));
+                    X[r * zs + nz + z] = gpe::mk(T(0), T(0));
+                }
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.pz2, rw, 1, zs, -1, a.tw);
+            for (int e = threadIdx.x; e < np; e += NT) {
+                const int r = e / nz, c = e - r * nz;
+                hs[(size_t)(y0 + r) * nz + c] = X[r * zs + c + 1];
+            }
+            __syncthreads();
+        }
+        // z kinetic of psi -> W (the G rows of this tile are consumed above)
+        if (mode & M_KIN) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int p = threadIdx.x + q * NT;
+                if (p < np) X[(p / nz) * qs + p % nz] = cv[q];
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.pz, rw, 1, qs, -1, a.tw, a.twlz);
+            for (int e = threadIdx.x; e < np; e += NT) {
+                const int qi = (e / nz) * qs + e % nz;
+                X[qi] = gpe::kin_mul(X[qi], a.ez[e % nz], 1.0 / nz);
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.pz, rw, 1, qs, +1, a.tw, a.twlz);
+            for (int e = threadIdx.x; e < np; e += NT) gw[(size_t)y0 * nz + e] = X[(e / nz) * qs + e % nz];
+            __syncthreads();
+        }
+    }
+
+    // ---- (C) column tiles: y2 of H -> S1o, y kinetic of W -> Tk
+    if (mode & M_RHO_FWD) {
+        const int cs = C + 1;
+        for (int c0 = 0; c0 < nz; c0 += C) {
+            const int cw = min(C, nz - c0);
+            for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
+                const int ky = e / cw, c = e - ky * cw;
+                X[ky * cs + c] = ky < ny ? hs[(size_t)ky * nz + c0 + c] : gpe::mk(T(0), T(0));
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.py2, cw, cs, 1, -1, a.tw);
+            for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
+                const int ky = e / cw, c = e - ky * cw;
+                a.s1o[((size_t)ky * nx + i) * nz + c0 + c] = X[ky * cs + c];
+            }
+            __syncthreads();
+        }
+    }
+    if (mode & M_KIN) {
+        const int cs = C + 1;
+        for (int c0 = 0; c0 < nz; c0 += C) {
+            const int cw = min(C, nz - c0);
+            for (int e = threadIdx.x; e < ny * cw; e += NT) {
+                const int y = e / cw, c = e - y * cw;
+                X[y * cs + c] = gw[(size_t)y * nz + c0 + c];
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.py, cw, cs, 1, -1, a.tw, a.twly);
+            for (int e = threadIdx.x; e < ny * cw; e += NT) {
+                const int y = e / cw, c = e - y * cw;
+                X[y * cs + c] = gpe::kin_mul(X[y * cs + c], a.ey[y], 1.0 / ny);
+            }
+            __syncthreads();
+            gpe::lds_fft<T, NT, MAXE>(X, a.py, cw, cs, 1, +1, a.tw, a.twly);
+            for (int e = threadIdx.x; e < ny * cw; e += NT) {
+                const int y = e / cw, c = e - y * cw;
+                a.tko[((size_t)y * nx + i) * nz + c0 + c] = X[y * cs + c];
+            }
+            __syncthreads();
+        }
+    }
+
+    if (mode & M_NORM_OUT) {
+        const double s = block_sum<NT>(nacc, red);
+        if (threadIdx.x == 0) a.part_out[i] = s;
+    }
+}
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_plane_pencil(PlaneArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[NT / 64 + 1];
+    pencil_body<T, NT>(a, blockIdx.x, a.mode, lds_raw, red);
+}
+
+// ---------------------------------------------------------------------------
 // Line kernel: x-direction passes.  Dynamic LDS: max(2nx*cp, nx*ck) complex T.
 template <typename T, int NT, class Pol>
 __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned char* lds_raw) {
@@ -1519,6 +1807,11 @@ struct gpe_handle {
     bool force_generic = false; // GPE_FORCE_GENERIC=1: run-time-size kernels only
     bool line_reg = true;       // register-resident x pass (GPE_LINE_LDS=1 disables)
     bool plane_split = true;    // two blocks per plane in steady real time (GPE_NO_SPLIT=1 disables)
+    bool pencil = false;        // plane beyond one block's LDS: the pencil plane pass
+    void* pen_gw = nullptr;     // its per-plane intermediates (N complex each)
+    void* pen_h = nullptr;
+    int pen_c = 0, pen_r = 0;   // column / row tile shapes
+    size_t pen_lds = 0;
     LinePlan px, py, pz, px2, py2, pz2;
     // launch shapes
     int cp = 0, ck = 0;
@@ -1761,6 +2054,10 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.ey = (const cplx<T>*)h->ey;
     a.twly = (const cplx<T>*)h->twl + p.nx;
     a.twlz = (const cplx<T>*)h->twl + p.nx + p.ny;
+    a.pen_gw = (cplx<T>*)h->pen_gw;
+    a.pen_h = (cplx<T>*)h->pen_h;
+    a.pen_c = h->pen_c;
+    a.pen_r = h->pen_r;
     a.pz2 = h->pz2;
     a.py2 = h->py2;
     a.pz = h->pz;
@@ -1851,6 +2148,14 @@ static int launch_plane_pol(gpe_handle* h, const PlaneArgs<T>& a) {
 template <typename T>
 static int launch_plane(gpe_handle* h, const PlaneArgs<T>& a) {
     const int ny = h->p.ny, nz = h->p.nz;
+    if (h->pencil) {
+        hipEvent_t e0, e1;
+        kernel_events(h, 0, &e0, &e1);
+        hipExtLaunchKernelGGL((k_plane_pencil<T, kPencilThreads>), dim3(h->p.nx), dim3(kPencilThreads), h->pen_lds,
+                              h->stream, e0, e1, 0, a);
+        GPE_CHECK(hipGetLastError());
+        return 0;
+    }
     if (!h->force_generic) {
 #define GPE_PL(NY, NZ) \
     if (ny == NY && nz == NZ) return launch_plane_pol<T, PlanePolCT<NY, NZ>>(h, a);
@@ -2495,10 +2800,18 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
         return fail(nullptr, "spacings, mass and n_atoms must be positive");
     const int maxe = precision == GPE_FP64 ? Lim<double>::maxe : Lim<float>::maxe;
     const long pe = 2L * p->ny * (p->nz + 1);
-    if (pe > maxe || (long)p->ny * p->nz > kMaxPlanePts)
+    // planes beyond one block's LDS run the pencil pass: its column tiles
+    // hold 2ny x 2 and its row tiles 2nz+1 complex elements at least
+    // GPE_FORCE_PENCIL=1 runs the pencil pass on any grid (tests compare the
+    // two plane passes on small grids)
+    const char* fpen = getenv("GPE_FORCE_PENCIL");
+    const bool pencil = pe > maxe || (long)p->ny * p->nz > kMaxPlanePts || (fpen && fpen[0] == '1');
+    const int pmaxe = precision == GPE_FP64 ? PencilLim<double>::maxe : PencilLim<float>::maxe;
+    if (pencil && (4L * p->ny > pmaxe || 2L * p->nz + 1 > pmaxe || p->nz > kPencilPPT * kPencilThreads))
         return fail(nullptr, "grid plane " + std::to_string(p->ny) + "x" + std::to_string(p->nz) +
-                                 " exceeds the single-block plane path at this precision");
+                                 " exceeds the pencil plane path at this precision");
     gpe_handle* h = new gpe_handle();
+    h->pencil = pencil;
     h->p = *p;
     h->prec = precision;
     h->N = (size_t)p->nx * p->ny * p->nz;
@@ -2513,7 +2826,7 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
         const char* ll = getenv("GPE_LINE_LDS");
         h->line_reg = !(ll && ll[0] == '1');
         const char* ns = getenv("GPE_NO_SPLIT");
-        h->plane_split = !(ns && ns[0] == '1');
+        h->plane_split = !(ns && ns[0] == '1') && !pencil;
     }
     auto bad = [&](const std::string& m) {
         g_create_err = m;
@@ -2544,6 +2857,19 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     h->plane_lds = (size_t)std::max(2L * p->ny * (p->nz + 1), (long)p->ny * (2 * p->nz + 1)) * cs;
     h->plane_stage = (size_t)(h->tw_n + 2 * (p->ny + p->nz)) * cs;
     h->line_lds = (size_t)std::max(2L * p->nx * (h->cp + 1), (long)p->nx * (h->ck + 1)) * cs;
+    if (pencil) {
+        h->pen_c = std::min(p->nz, pmaxe / (2 * p->ny) - 1);
+        h->pen_r = std::min(std::min(p->ny, kPencilPPT * kPencilThreads / p->nz), pmaxe / (2 * p->nz + 1));
+        // GPE_PENCIL_TILES="c,r" caps the tile shapes (tests: partial last tiles)
+        const char* pt = getenv("GPE_PENCIL_TILES");
+        int tc = 0, tr = 0;
+        if (pt && sscanf(pt, "%d,%d", &tc, &tr) == 2 && tc > 0 && tr > 0) {
+            h->pen_c = std::min(h->pen_c, tc);
+            h->pen_r = std::min(h->pen_r, tr);
+        }
+        h->pen_lds = (size_t)std::max(2L * p->ny * (h->pen_c + 1), (long)h->pen_r * (2 * p->nz + 1)) * cs;
+        if (hipMalloc(&h->pen_gw, n * cs) || hipMalloc(&h->pen_h, n * cs)) return bad("device allocation failed");
+    }
     if (hipDeviceSynchronize() != hipSuccess) return bad("device synchronize failed after setup");
     *out = h;
     return 0;
@@ -2554,7 +2880,7 @@ void gpe_destroy(gpe_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->store[0], h->store[1], h->psii,     h->V,       h->phi,    h->s1,  h->tk,
                     h->s1b,      h->tkb,      h->tw,       h->twl,     h->ex,     h->ey,  h->ez,  h->kx2,
-                    h->ky2,      h->kz2,      h->part[0],  h->part[1], h->epart};
+                    h->ky2,      h->kz2,      h->part[0],  h->part[1], h->epart,  h->pen_gw, h->pen_h};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
@@ -2646,6 +2972,11 @@ int gpe_batch_create(const gpe_params* params, int n, int precision, gpe_batch**
         if (gpe_create(&params[q], precision, &h) != 0) {
             gpe_batch_destroy(b);
             return 1;
+        }
+        if (h->pencil) {
+            gpe_destroy(h);
+            gpe_batch_destroy(b);
+            return fail(nullptr, "batched instances need planes of the single-block plane path");
         }
         (void)hipStreamDestroy(h->stream);
         h->stream = b->stream;
@@ -2765,7 +3096,7 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
         int rc = gpe_load_buffer(s, psi.data());
         const bool reg_ok = !s->force_generic && (h->p.nx == 128 || h->p.nx == 64);
         double best_cost = 0;
-        for (int sp = 1; sp >= 0 && !rc; --sp)
+        for (int sp = s->pencil ? 0 : 1; sp >= 0 && !rc; --sp)
             for (int lr = reg_ok ? 1 : 0; lr >= 0 && !rc; --lr) {
                 s->plane_split = sp != 0;
                 s->line_reg = lr != 0;

@@ -47,9 +47,9 @@ def test_create_rejects_bad_arguments_without_touching_the_device():
     assert lib.gpe_create(C.byref(g), 64, C.byref(h)) != 0
     assert b"positive" in lib.gpe_last_error(None)
     g.n_atoms = 1.0
-    g.nz = 2048  # plane too large for the single-block plane path
+    g.nz = 4096  # z rows beyond the row tiles of the pencil plane pass
     assert lib.gpe_create(C.byref(g), 32, C.byref(h)) != 0
-    assert b"plane" in lib.gpe_last_error(None)
+    assert b"pencil plane path" in lib.gpe_last_error(None)
 
 
 def test_null_handle_calls_fail_cleanly():

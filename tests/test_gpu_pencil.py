@@ -1,0 +1,169 @@
+"""The pencil plane pass (planes whose Poisson tile exceeds one block's LDS,
+such as 256x256x64): one block per x-plane sweeping LDS tiles, with the
+plane-sized intermediates in HBM.
+
+* Small grids with GPE_FORCE_PENCIL=1 (and small tiles through
+  GPE_PENCIL_TILES, so the last column / row tiles are partial): every hook
+  and the fused IT / RT loops against the FP64 oracle, and against the
+  single-block plane pass on the same device.
+* 256x256x64 (dx = dy = 60 nm, dz = 300 nm, 1e5 atoms, MEXICAN_ASYMETRIC,
+  SETUP_GAUSS with 3x3x2 droplets, edd ramped 1.5 -> 1.3 over 20000 steps):
+  the first real-time steps against the oracle, then 500 steps on the FP32
+  and FP64 paths with the norm checked every 100 steps; the FP32 norm drift
+  and energy error against FP64 are printed and bounded.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gpesolver_amd import hip
+from gpesolver_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+TOL_PSI = {hip.GPE_FP64: 1e-10, hip.GPE_FP32: 1e-5}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def small_case(oracle, shape, pot="HARMONIC", init="GAUSS", n_atoms=500.0):
+    nx, ny, nz = shape
+    d = (120.0, 120.0, 600.0)
+    dd = 0.2 * nx * d[0]
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, *d, n_atoms=n_atoms, edd=1.5, dd_nm=dd)
+    op = oracle.make_params(nx, ny, nz, *d, n_atoms=n_atoms, edd=1.5, dd_nm=dd)
+    V = P.potential(pot, p)
+    psi = P.add_phase_noise(P.wavefunction(init, p, n_gauss_max=2), 1e-3, 5)
+    return p, op, V, psi
+
+
+def dev_of(p, prec, V, psi):
+    d = hip.Device(p.to_gpe(), prec)
+    d.load_pote(V)
+    d.load_buffer(psi)
+    return d
+
+
+@pytest.mark.parametrize("shape,tiles", [((24, 20, 12), "3,5"), ((16, 16, 8), ""), ((20, 10, 6), "4,3"),
+                                         ((28, 14, 10), "2,4")])
+def test_pencil_pass_against_oracle_and_plane_pass(oracle, shape, tiles, monkeypatch):
+    print("pencil", shape, tiles, flush=True)
+    p, op, V, psi = small_case(oracle, shape)
+    for prec in (hip.GPE_FP64, hip.GPE_FP32):
+        monkeypatch.delenv("GPE_FORCE_PENCIL", raising=False)
+        monkeypatch.delenv("GPE_PENCIL_TILES", raising=False)
+        ref = dev_of(p, prec, V, psi)  # single-block plane pass
+        monkeypatch.setenv("GPE_FORCE_PENCIL", "1")
+        if tiles:
+            monkeypatch.setenv("GPE_PENCIL_TILES", tiles)
+        dev = dev_of(p, prec, V, psi)
+        orc = oracle.Oracle(op)
+        orc.load_pote(V)
+        orc.load_buffer(psi)
+        tol = TOL_PSI[prec]
+        # Poisson
+        dev.calc_fi3d()
+        ref.calc_fi3d()
+        orc.calc_fi3d()
+        assert rel(dev.export_fi3d(), orc.fi3d) < (1e-12 if prec == hip.GPE_FP64 else 1e-5)
+        # fused imaginary time
+        dev.it_steps(10)
+        ref.it_steps(10)
+        orc.it_steps(10)
+        assert rel(dev.export_psi(), orc.psi) < tol
+        assert rel(dev.export_psi(), ref.export_psi()) < (1e-13 if prec == hip.GPE_FP64 else 1e-6)
+        e, eo = dev.energy(), orc.energy()
+        assert abs(e["e_total"] - eo["e_total"]) < (1e-10 if prec == hip.GPE_FP64 else 1e-6) * abs(eo["e_total"])
+        # kinetic hook and fused real time from the oracle's state
+        orc.calc_fi3d()
+        for d in (dev, ref):
+            d.load_buffer(orc.psi)
+            d.import_fi3d(orc.fi3d.copy())
+        dev.kinetic_step()
+        ref.kinetic_step()
+        orc.kinetic_step()
+        assert rel(dev.export_psi(), orc.psi) < tol
+        for d in (dev, ref):
+            d.load_buffer(orc.psi)
+        dev.rt_steps(4)
+        ref.rt_steps(4)
+        orc.rt_steps(0, 4)
+        assert rel(dev.export_psi(), orc.psi) < (tol if prec == hip.GPE_FP64 else 3e-6)
+        assert rel(dev.export_psi(), ref.export_psi()) < (1e-13 if prec == hip.GPE_FP64 else 2e-6)
+        assert rel(dev.export_fi3d(), orc.fi3d) < (1e-11 if prec == hip.GPE_FP64 else 1e-5)
+        dev.close()
+        ref.close()
+
+
+LARGE = (256, 256, 64)
+D_LARGE = (60.0, 60.0, 300.0)
+
+
+def large_case(oracle=None, n_ramp=20000):
+    nx, ny, nz = LARGE
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, *D_LARGE, n_atoms=1e5, edd=None, edd_start=1.5,
+                                           edd_stop=1.3, iter_real=n_ramp)
+    V = P.potential("MEXICAN_ASYMETRIC", p)
+    psi = P.wavefunction("SETUP_GAUSS", p, drops=(3, 3, 2))
+    op = None
+    if oracle is not None:
+        op = oracle.make_params(nx, ny, nz, *D_LARGE, n_atoms=1e5, edd=None, edd_start=1.5, edd_stop=1.3,
+                                iter_real=n_ramp)
+    return p, op, V, psi
+
+
+def couplings(p, t0, n):
+    g = np.array([p.coupling_at(t)[1] for t in range(t0, t0 + n)])
+    gm = np.array([p.coupling_at(t)[2] for t in range(t0, t0 + n)])
+    return g, gm
+
+
+def test_large_rt_first_steps_against_oracle(oracle):
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    p, op, V, psi = large_case(oracle)
+    orc = oracle.Oracle(op)
+    orc.load_pote(V)
+    orc.load_buffer(psi)
+    orc.rt_steps(0, 3)
+    g, gm = couplings(p, 0, 3)
+    for prec in (hip.GPE_FP64, hip.GPE_FP32):
+        dev = dev_of(p, prec, V, psi)
+        dev.rt_steps(3, g, gm)
+        r = rel(dev.export_psi(), orc.psi)
+        print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} 3 RT steps vs oracle: psi {r:.3e}",
+              flush=True)
+        assert r < (1e-10 if prec == hip.GPE_FP64 else 2e-6)
+        dev.close()
+
+
+def test_large_rt_500_steps_fp32_vs_fp64():
+    p, _, V, psi = large_case()
+    devs = {prec: dev_of(p, prec, V, psi) for prec in (hip.GPE_FP64, hip.GPE_FP32)}
+    e0 = {prec: d.energy()["e_total"] for prec, d in devs.items()}
+    worst_norm = 0.0
+    worst_e = 0.0
+    for t0 in range(0, 500, 100):
+        g, gm = couplings(p, t0, 100)
+        for d in devs.values():
+            d.rt_steps(100, g, gm)
+        n64 = devs[hip.GPE_FP64].calc_norm()
+        n32 = devs[hip.GPE_FP32].calc_norm()
+        e64 = devs[hip.GPE_FP64].energy()["e_total"]
+        e32 = devs[hip.GPE_FP32].energy()["e_total"]
+        de = abs(e32 - e64) / abs(e64)
+        print(f"step {t0 + 100}: norm-1 FP64 {n64 - 1:+.2e} FP32 {n32 - 1:+.2e}  E/N FP32 vs FP64 {de:.2e}  "
+              f"E/N {e64 / p.n_atoms:.9e}", flush=True)
+        assert abs(n64 - 1.0) < 1e-11
+        worst_norm = max(worst_norm, abs(n32 - 1.0))
+        worst_e = max(worst_e, de)
+    psi_err = rel(devs[hip.GPE_FP32].export_psi(), devs[hip.GPE_FP64].export_psi())
+    print(f"after 500 steps: FP32 psi vs FP64 {psi_err:.2e}, max norm drift {worst_norm:.2e}, "
+          f"max E/N error {worst_e:.2e} (E/N at t=0: {e0[hip.GPE_FP64] / p.n_atoms:.9e})", flush=True)
+    assert worst_norm < 1e-6
+    assert worst_e < 1e-5
+    assert psi_err < 1e-4
+    for d in devs.values():
+        d.close()
