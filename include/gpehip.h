@@ -53,7 +53,9 @@ typedef struct {
 
 /* Create / destroy.  Replaces GpuGrossPitaevskiSolver::GpuGrossPitaevskiSolver +
  * init_containers + prepare_fft (src/solver/cuda_solver/gpu_solver.cpp:4-38,
- * src/solver/solver.cpp:15-23). */
+ * src/solver/solver.cpp:15-23).  Grid lengths of the form 2^a 3^b 5^c 7^d;
+ * planes whose Poisson tile exceeds one block's LDS (256x256x64, say) run
+ * the pencil plane pass, with two extra complex fields of workspace. */
 int gpe_create(const gpe_params *p, int precision, gpe_handle **out);
 void gpe_destroy(gpe_handle *h);
 /* Message of the last failed call on h (h == NULL: last failed gpe_create). */
