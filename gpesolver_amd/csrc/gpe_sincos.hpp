@@ -55,6 +55,28 @@ __device__ __forceinline__ cplx<float> phase_rot(cplx<float> c, double theta) {
     return o;
 }
 
+// c * exp(i r) for a small FP32 phase, |r| <= pi/4 (no reduction): the
+// increment form of phase_rot.
+__device__ __forceinline__ cplx<float> phase_rot_small(cplx<float> c, float r) {
+    const float u = r * r;
+    float ps = SinCosPoly::S[3];
+    ps = ps * u + SinCosPoly::S[2];
+    ps = ps * u + SinCosPoly::S[1];
+    ps = ps * u + SinCosPoly::S[0];
+    float pc = SinCosPoly::C[3];
+    pc = pc * u + SinCosPoly::C[2];
+    pc = pc * u + SinCosPoly::C[1];
+    pc = pc * u + SinCosPoly::C[0];
+    const float sr = r + r * u * ps;
+    const float cm1 = u * pc;
+    const float dx = fmaf(c.x, cm1, -(c.y * sr));
+    const float dy = fmaf(c.y, cm1, c.x * sr);
+    cplx<float> o;
+    o.x = c.x + dx;
+    o.y = c.y + dy;
+    return o;
+}
+
 // FP64: the FP64 sincos of the reduced phase.
 __device__ __forceinline__ cplx<double> phase_rot(cplx<double> c, double theta) {
     double s, co;

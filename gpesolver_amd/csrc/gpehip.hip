@@ -902,6 +902,11 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         }
         __syncthreads();
         const bool iint = (i >= 1 && i <= nx - 2);
+        // FP32: the phase argument in FP32 where it is small (|theta| <= 1/16,
+        // absolute error ~1e-9 rad, no reduction needed); larger phases take
+        // the FP64 argument and quarter-turn reduction
+        const float tv = (float)(a.hdt * a.cv), tphi = (float)(a.hdt * a.cv * a.cdd), tg = (float)(a.hdt * pg),
+                    tgm = (float)(a.hdt * pgm);
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int p = threadIdx.x + q * NT;
@@ -912,13 +917,24 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 if (mode & (M_PHASE_HALF | M_PHASE_FULL)) {
                     const int j = p / nz, k = p - j * nz;
                     if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
-                        // phase argument in FP64, reduced in FP64 before the
-                        // sin / cos of the bulk precision, so a large phase (dense
-                        // states, long steps) keeps its absolute accuracy
-                        const double dn = (double)nrm;
-                        const double vv = kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
-                        const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
-                        c = gpe::phase_rot(c, -a.hdt * veff);
+                        bool done = false;
+                        if constexpr (sizeof(T) == 4) {
+                            const float vv = kStage ? vpre[q < PRE ? q : 0] : a.V[plane_off + p];
+                            const float t32 = tv * vv + tphi * phi_r[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
+                            if (fabsf(t32) <= 0.0625f) {
+                                c = gpe::phase_rot_small(c, -t32);
+                                done = true;
+                            }
+                        }
+                        if (!done) {
+                            // phase argument in FP64, reduced in FP64 before the
+                            // sin / cos of the bulk precision, so a large phase (dense
+                            // states, long steps) keeps its absolute accuracy
+                            const double dn = (double)nrm;
+                            const double vv = kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
+                            const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
+                            c = gpe::phase_rot(c, -a.hdt * veff);
+                        }
                         Q[qi] = c;
                         nrm = c.x * c.x + c.y * c.y;
                     }
