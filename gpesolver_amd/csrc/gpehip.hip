@@ -471,7 +471,7 @@ __device__ __forceinline__ void kin_y_reg_store(cplx<T>* Q, int qs, const cplx<T
         for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * ES + c];
         gpe::ct::Dft<T, A, +1, true>::run(v);
 #pragma unroll
-        for (int q = 0; q < A; ++q) tk_plane[(size_t)(B * q + b) * nx * NZ + c] = v[q];
+        for (int q = 0; q < A; ++q) store_nt(&tk_plane[(size_t)(B * q + b) * nx * NZ + c], v[q]);
     }
 }
 
@@ -581,7 +581,7 @@ __device__ __forceinline__ void y2_fwd_reg_store(cplx<T>* P, int zs, const cplx<
             for (int bb = 0; bb < Y::B; ++bb) w[bb] = P[(k1 * Y::B + bb) * Y::ES + c];
             gpe::ct::Dft<T, Y::B, -1>::run(w);
 #pragma unroll
-            for (int k2 = 0; k2 < Y::B; ++k2) s1_plane[(size_t)(k1 + 16 * k2) * nx * NZ + c] = w[k2];
+            for (int k2 = 0; k2 < Y::B; ++k2) store_nt(&s1_plane[(size_t)(k1 + 16 * k2) * nx * NZ + c], w[k2]);
         }
     }
 }
