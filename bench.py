@@ -221,7 +221,7 @@ def main():
             "n_atoms": 2e4,
             "edd": 1.5,
             "parallelism": "single GPU",
-            "tuned": {k: tuned[k] for k in ("split_plane_pass", "register_x_pass")},
+            "tuned": {k: tuned[k] for k in ("split_plane_pass", "register_x_pass", "graph_replay")},
         },
         "roofline": {
             "bound": "hbm",

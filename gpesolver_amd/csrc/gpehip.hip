@@ -3059,7 +3059,7 @@ static void drop_graphs(gpe_handle* h) {
 }
 
 struct TuneResult {
-    bool split, line_reg;
+    bool split, line_reg, graphs;
     double rt_us, it_us;
 };
 static std::mutex g_tune_mu;
@@ -3111,21 +3111,26 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
         std::vector<double> psi(2 * h->N, 1.0);
         int rc = gpe_load_buffer(s, psi.data());
         const bool reg_ok = !s->force_generic && (h->p.nx == 128 || h->p.nx == 64);
+        // graph replay of the steady state, or direct launches (GPE_NO_GRAPH=1
+        // leaves only the latter): which one is faster depends on the grid
+        const bool graphs_ok = s->use_graphs;
         double best_cost = 0;
         for (int sp = s->pencil ? 0 : 1; sp >= 0 && !rc; --sp)
-            for (int lr = reg_ok ? 1 : 0; lr >= 0 && !rc; --lr) {
-                s->plane_split = sp != 0;
-                s->line_reg = lr != 0;
-                drop_graphs(s);
-                double rt = 0, itu = 0;
-                rc = tune_time(s, steps, &rt, &itu);
-                const double cost = rt + itu;
-                if (!rc && (!have || cost < best_cost)) {
-                    best = TuneResult{sp != 0, lr != 0, rt, itu};
-                    best_cost = cost;
-                    have = true;
+            for (int lr = reg_ok ? 1 : 0; lr >= 0 && !rc; --lr)
+                for (int gr = graphs_ok ? 1 : 0; gr >= 0 && !rc; --gr) {
+                    s->plane_split = sp != 0;
+                    s->line_reg = lr != 0;
+                    s->use_graphs = gr != 0;
+                    drop_graphs(s);
+                    double rt = 0, itu = 0;
+                    rc = tune_time(s, steps, &rt, &itu);
+                    const double cost = rt + itu;
+                    if (!rc && (!have || cost < best_cost)) {
+                        best = TuneResult{sp != 0, lr != 0, gr != 0, rt, itu};
+                        best_cost = cost;
+                        have = true;
+                    }
                 }
-            }
         if (rc) {
             const std::string m = s->err;
             gpe_destroy(s);
@@ -3138,7 +3143,9 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
     if (h->plane_split != best.split || h->line_reg != best.line_reg) drop_graphs(h);
     h->plane_split = best.split;
     h->line_reg = best.line_reg;
-    if (config) *config = (best.split ? 1 : 0) | (best.line_reg ? 2 : 0);
+    const char* ng = getenv("GPE_NO_GRAPH");
+    h->use_graphs = best.graphs && !(ng && ng[0] == '1');
+    if (config) *config = (best.split ? 1 : 0) | (best.line_reg ? 2 : 0) | (h->use_graphs ? 4 : 0);
     if (rt_step_us) *rt_step_us = best.rt_us;
     if (it_step_us) *it_step_us = best.it_us;
     return 0;

@@ -232,7 +232,7 @@ class Device:
         rt, it = C.c_double(), C.c_double()
         self._check(self.L.gpe_autotune(self.h, steps, C.byref(cfg), C.byref(rt), C.byref(it)))
         return {"split_plane_pass": bool(cfg.value & 1), "register_x_pass": bool(cfg.value & 2),
-                "rt_step_us": rt.value, "it_step_us": it.value}
+                "graph_replay": bool(cfg.value & 4), "rt_step_us": rt.value, "it_step_us": it.value}
 
     def set_timing(self, on: bool):
         self._check(self.L.gpe_set_timing(self.h, 1 if on else 0))

@@ -144,10 +144,11 @@ int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float
 
 /* Autotuner (no reference counterpart).  Times the launch-shape variants of
  * the fused loops (one or two blocks per plane in the real-time plane pass;
- * register-resident or LDS x pass) over `steps` steps on a scratch instance
- * of h's grid and precision, and keeps the fastest for h (cached per grid
- * and precision for the process).  config receives bit 0 = split plane pass,
- * bit 1 = register x pass; rt_step_us / it_step_us the tuned step times.
+ * register-resident or LDS x pass; graph-replayed or directly launched
+ * steady state) over `steps` steps on a scratch instance of h's grid and
+ * precision, and keeps the fastest for h (cached per grid and precision for
+ * the process).  config receives bit 0 = split plane pass, bit 1 = register
+ * x pass, bit 2 = graph replay; rt_step_us / it_step_us the tuned step times.
  * Every variant computes the same transforms; the two x passes round
  * differently, so a tuned handle is deterministic but may differ in the last
  * bits from an untuned one (defaults: split, register x pass). */

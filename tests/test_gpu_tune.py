@@ -39,6 +39,7 @@ def test_autotune_matches_forced_variant(monkeypatch, prec):
     d.close()
     monkeypatch.setenv("GPE_NO_SPLIT", "0" if cfg["split_plane_pass"] else "1")
     monkeypatch.setenv("GPE_LINE_LDS", "0" if cfg["register_x_pass"] else "1")
+    monkeypatch.setenv("GPE_NO_GRAPH", "0" if cfg["graph_replay"] else "1")
     f = hip.Device(p.to_gpe(), prec)
     forced = run(f, V, psi)
     f.close()
