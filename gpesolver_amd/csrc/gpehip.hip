@@ -2799,6 +2799,30 @@ static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double
     return 0;
 }
 
+template <typename T>
+static int do_it_until(gpe_handle* h, long max_steps, long check_every, double tol, long* steps_out,
+                       double* e_out) {
+    if (check_every <= 0) check_every = 100;
+    long steps = 0;
+    double last = 0.0, epa = 0.0;
+    bool have = false;
+    while (steps < max_steps) {
+        const long n = std::min(check_every, max_steps - steps);
+        if (do_it_iters<T>(h, n)) return 1;
+        steps += n;
+        gpe_energies e;
+        if (do_energy<T>(h, &e)) return 1;
+        epa = e.e_total / h->p.n_atoms;
+        const bool conv = tol > 0 && have && std::abs(epa - last) < tol * std::abs(epa);
+        last = epa;
+        have = true;
+        if (conv) break;
+    }
+    if (steps_out) *steps_out = steps;
+    if (e_out) *e_out = epa;
+    return 0;
+}
+
 #define DISPATCH(h, fn, ...) ((h)->prec == GPE_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
 extern "C" {
@@ -2962,6 +2986,12 @@ int gpe_calc_energy(gpe_handle* h, gpe_energies* out) {
     return DISPATCH(h, do_energy, h, out);
 }
 int gpe_imag_time_iters(gpe_handle* h, long n) { return h ? DISPATCH(h, do_it_iters, h, n) : 1; }
+int gpe_imag_time_until(gpe_handle* h, long max_steps, long check_every, double tol, long* steps_out,
+                        double* energy_per_atom_out) {
+    if (!h) return 1;
+    if (max_steps < 0) return fail(h, "max_steps must be >= 0");
+    return DISPATCH(h, do_it_until, h, max_steps, check_every, tol, steps_out, energy_per_atom_out);
+}
 int gpe_real_time_iters(gpe_handle* h, long n, const double* g, const double* gm) {
     return h ? DISPATCH(h, do_rt_iters, h, n, g, gm) : 1;
 }

@@ -22,8 +22,8 @@ EXPORTS = (
     "gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d",
     "gpe_set_coupling", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy",
-    "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_synchronize", "gpe_set_timing", "gpe_get_timing",
-    "gpe_autotune",
+    "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_imag_time_until", "gpe_synchronize", "gpe_set_timing",
+    "gpe_get_timing", "gpe_autotune",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
     "gpe_fno_create", "gpe_fno_destroy", "gpe_fno_last_error", "gpe_fno_num_params", "gpe_fno_get_params",
     "gpe_fno_set_params", "gpe_fno_get_grads", "gpe_fno_forward", "gpe_fno_loss_grad", "gpe_fno_adam_step",
@@ -89,6 +89,8 @@ def load(path: str = LIB_PATH):
     L.gpe_calc_energy.restype = ip
     L.gpe_imag_time_iters.argtypes = [vp, C.c_long]
     L.gpe_imag_time_iters.restype = ip
+    L.gpe_imag_time_until.argtypes = [vp, C.c_long, C.c_long, C.c_double, C.POINTER(C.c_long), dp]
+    L.gpe_imag_time_until.restype = ip
     L.gpe_real_time_iters.argtypes = [vp, C.c_long, dp, dp]
     L.gpe_real_time_iters.restype = ip
     L.gpe_autotune.argtypes = [vp, C.c_long, C.POINTER(C.c_int), dp, dp]
@@ -213,6 +215,13 @@ class Device:
     # loops
     def it_steps(self, n: int):
         self._check(self.L.gpe_imag_time_iters(self.h, n))
+
+    def it_until(self, max_steps: int, check_every: int = 100, tol: float = 1e-6) -> tuple[int, float]:
+        """Imaginary time until |dE/N| < tol |E/N| between checks (gpe_imag_time_until):
+        (steps run, last E/N)."""
+        st, e = C.c_long(), C.c_double()
+        self._check(self.L.gpe_imag_time_until(self.h, max_steps, check_every, tol, C.byref(st), C.byref(e)))
+        return st.value, e.value
 
     def rt_steps(self, n: int, ggp11=None, gamma=None):
         if ggp11 is None:

@@ -93,6 +93,14 @@ int gpe_calc_energy(gpe_handle *h, gpe_energies *out); /* cpu_solver.cpp:137-194
  * of src/solver/solver.cpp:76-77); NULL keeps the current coupling. */
 int gpe_imag_time_iters(gpe_handle *h, long n);
 int gpe_real_time_iters(gpe_handle *h, long n, const double *ggp11, const double *gamma);
+/* The calc_initial_state loop (src/solver/solver.cpp:48-70) with the energy
+ * check it prints every 100 steps (solver.cpp:58-60) used as a stop test:
+ * imaginary time until |E/N(k) - E/N(k - check_every)| < tol |E/N(k)| or
+ * max_steps (tol <= 0: max_steps).  steps_out / energy_per_atom_out (may be
+ * NULL) receive the steps run and the last E/N.  Same test as
+ * gpe_batch_it_until. */
+int gpe_imag_time_until(gpe_handle *h, long max_steps, long check_every, double tol, long *steps_out,
+                        double *energy_per_atom_out);
 
 /* Batch (no reference counterpart: the reference is single-instance with
  * singleton parameters, src/parameters/parameters.hpp:14-20).  n instances,

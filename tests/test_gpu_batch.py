@@ -73,3 +73,34 @@ def test_batch_convergence_drops_instances():
             e2 = d.energy()["e_total"]
             assert abs(e2 - e1) < 1e-6 * abs(e2) * 1.5
     b.close()
+
+
+@pytest.mark.parametrize("prec", [hip.GPE_FP64, hip.GPE_FP32], ids=["fp64", "fp32"])
+def test_single_instance_until_matches_batch_of_one(prec):
+    """gpe_imag_time_until on one handle stops where a batch of that one
+    instance stops (same test, same steps, same E/N), and max_steps caps it."""
+    p, V, psi = setup_instance((32, 24, 12), 1.4, 1e4)
+    d = hip.Device(p.to_gpe(), prec)
+    d.load_pote(V)
+    d.load_buffer(psi)
+    steps, e = d.it_until(4000, check_every=100, tol=1e-6)
+    b = hip.Batch([p.to_gpe()], prec)
+    b.instances[0].load_pote(V)
+    b.instances[0].load_buffer(psi)
+    bsteps, be = b.it_until(4000, check_every=100, tol=1e-6)
+    print("until", prec, steps, e, int(bsteps[0]), flush=True)
+    assert 0 < steps < 4000 and steps % 100 == 0
+    if prec == hip.GPE_FP64:
+        assert steps == int(bsteps[0])
+        assert abs(e - be[0]) <= 1e-13 * abs(e)
+        assert rel(d.export_psi(), b.instances[0].export_psi()) < 1e-13
+    else:
+        # the batched kernels round differently: the stop may fall one check apart
+        assert abs(steps - int(bsteps[0])) <= 100
+        assert abs(e - be[0]) <= 1e-6 * abs(e)
+    # the cap, and a check interval that does not divide it
+    d.load_buffer(psi)
+    steps2, _ = d.it_until(250, check_every=100, tol=0.0)
+    assert steps2 == 250
+    d.close()
+    b.close()
