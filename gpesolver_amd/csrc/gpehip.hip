@@ -77,7 +77,7 @@ constexpr int kLineMaxE = 4096;     // complex elements of one x-line tile
 
 // Plane shapes (ny, nz) and x lengths with compile-time kernels.
 #define GPE_PLANE_SHAPES(X) X(128, 32) X(64, 16) X(96, 24) X(16, 8) X(24, 12) X(12, 6)
-#define GPE_LINE_SHAPES(X) X(128) X(64) X(96) X(16) X(24) X(32)
+#define GPE_LINE_SHAPES(X) X(128) X(64) X(96) X(16) X(24) X(32) X(256)
 
 template <typename T>
 struct Lim;
@@ -146,6 +146,7 @@ struct PlaneArgs {
     cplx<T>* pen_h;   // H (z R2C of rho, kz = 1..nz)
     int pen_c;        // columns per column tile
     int pen_r;        // rows per row tile
+    double* pen_part; // FP64 norm partial per (plane, row tile)
 };
 
 template <typename T>
@@ -1072,14 +1073,15 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
 
 // ---------------------------------------------------------------------------
 // Pencil plane pass: planes whose Poisson tile (2ny x (nz+1) complex) does
-// not fit one block's LDS.  One block per x-plane runs the modes of
-// plane_body in three sweeps over LDS tiles, with the plane-sized
-// intermediates in HBM (written and read back by the same block; most of it
-// stays in L2):
-//   (A) column tiles: inverse y2 of S1 -> G[y][kz-1], y < ny
-//   (B) row tiles: z C2R of G -> Phi; psi update (IT stencil or RT phase);
-//       z R2C of rho -> H[y][kz-1]; z kinetic of psi -> W (over G's rows)
-//   (C) column tiles: y2 of H -> S1o; y kinetic of W -> Tk
+// not fit one block's LDS.  The modes of plane_body run as three launches
+// over LDS tiles, one block per (x-plane, tile), with the plane-sized
+// intermediates in HBM:
+//   k_pencil_ci (column tiles): inverse y2 of S1 -> G[y][kz-1], y < ny
+//   k_pencil_r  (row tiles): z C2R of G -> Phi; psi update (IT stencil or RT
+//       phase); z R2C of rho -> H[y][kz-1]; z kinetic of psi -> W (over G's
+//       rows); FP64 norm partial per tile
+//   k_pencil_cf (column tiles): y2 of H -> S1o; y kinetic of W -> Tk; the
+//       per-plane norm partial as a fixed-order sum of the tile partials
 // Same arithmetic as plane_body's run-time-size path.
 constexpr int kPencilThreads = 512;
 constexpr int kPencilPPT = 4;  // row-tile points per thread
@@ -1095,261 +1097,327 @@ struct PencilLim<double> {
     static constexpr int maxe = 4096;
 };
 
-template <typename T, int NT>
-__device__ __forceinline__ void pencil_body(const PlaneArgs<T>& a, const int i, const int mode,
-                                            unsigned char* lds_raw, double* red) {
-    constexpr int MAXE = PencilLim<T>::maxe;
-    constexpr int PPT = kPencilPPT;
-    cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int ny2 = 2 * ny, nz2 = 2 * nz, npl = ny * nz;
-    const size_t plane_off = (size_t)i * npl;
-    const bool is_it = (mode & M_IT) != 0;
-    const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
-    const int C = a.pen_c, R = a.pen_r;
-    cplx<T>* gw = a.pen_gw + plane_off;  // [y][c]
-    cplx<T>* hs = a.pen_h + plane_off;   // [y][kz - 1]
+// Tile shapes and line FFTs of the pencil kernels: run-time sizes (radix-4/2/
+// 3/5/7 engine, tiles from the arguments) or compile-time shapes (large-radix
+// engine, fixed tiles of C columns and R rows).
+struct PencilPolGeneric {
+    template <class A>
+    __device__ static int C(const A& a) { return a.pen_c; }
+    template <class A>
+    __device__ static int R(const A& a) { return a.pen_r; }
+    template <typename T, int NT, int S>
+    __device__ static void y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, PencilLim<T>::maxe>(b, a.py2, nl, es, ls, S, a.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, PencilLim<T>::maxe>(b, a.pz2, nl, es, ls, S, a.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, PencilLim<T>::maxe>(b, a.py, nl, es, ls, S, a.tw, a.twly);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::lds_fft<T, NT, PencilLim<T>::maxe>(b, a.pz, nl, es, ls, S, a.tw, a.twlz);
+    }
+};
 
-    double pg = a.pg, pgm = a.pgm;
+template <int NY, int NZ, int CC, int RR>
+struct PencilPolCT {
+    static constexpr int kC = CC, kR = RR;
+    template <class A>
+    __device__ static constexpr int C(const A&) { return CC; }
+    template <class A>
+    __device__ static constexpr int R(const A&) { return RR; }
+    template <typename T, int NT, int S>
+    __device__ static void y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, 2 * NY, NT, CC, S>(b, nl, es, ls, a.tw + a.py2.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void z2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, 2 * NZ, NT, RR, S>(b, nl, es, ls, a.tw + a.pz2.tw);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void y(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, NY, NT, CC, S, true>(b, nl, es, ls, a.tw + a.py.tw, a.twly);
+    }
+    template <typename T, int NT, int S>
+    __device__ static void z(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
+        gpe::ct::fft<T, NZ, NT, RR, S, true>(b, nl, es, ls, a.tw + a.pz.tw, a.twlz);
+    }
+};
+// Pencil shapes with compile-time kernels: (ny, nz, columns, rows per tile).
+#define GPE_PENCIL_SHAPES(X) X(256, 64, 8, 16)
+
+// phase couplings of this step (as in plane_body)
+template <typename T>
+__device__ __forceinline__ void pencil_couplings(const PlaneArgs<T>& a, int mode, double* pg, double* pgm) {
+    *pg = a.pg;
+    *pgm = a.pgm;
     if (a.coef && (mode & (M_PHASE_HALF | M_PHASE_FULL))) {
         const int t = (a.cbase ? *a.cbase : 0) + a.cidx;
-        pg = a.coef[2 * t];
-        pgm = a.coef[2 * t + 1];
+        *pg = a.coef[2 * t];
+        *pgm = a.coef[2 * t + 1];
         if (mode & M_PHASE_FULL) {
-            pg += a.coef[2 * t + 2];
-            pgm += a.coef[2 * t + 3];
+            *pg += a.coef[2 * t + 2];
+            *pgm += a.coef[2 * t + 3];
         }
-    }
-    double s_in = 1.0;
-    if (mode & M_NORM_IN) s_in = 1.0 / sqrt(sum_partials(a.part_in, nx, red));
-
-    // ---- (A) inverse y2 of the S1 columns (kz = 1..nz) -> G
-    if (mode & M_PINV) {
-        const int cs = C + 1;
-        for (int c0 = 0; c0 < nz; c0 += C) {
-            const int cw = min(C, nz - c0);
-            for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
-                const int ky = e / cw, c = e - ky * cw;
-                X[ky * cs + c] = a.s1[((size_t)ky * nx + i) * nz + c0 + c];
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.py2, cw, cs, 1, +1, a.tw);
-            for (int e = threadIdx.x; e < ny * cw; e += NT) {
-                const int y = e / cw, c = e - y * cw;
-                gw[(size_t)y * nz + c0 + c] = X[y * cs + c];
-            }
-            __syncthreads();
-        }
-    }
-
-    // ---- (B) row tiles
-    double nacc = 0.0;
-    const int zs = nz2 + 1, qs = nz + 1;
-    const bool iint = (i >= 1 && i <= nx - 2);
-    for (int y0 = 0; y0 < ny; y0 += R) {
-        const int rw = min(R, ny - y0);
-        const int np = rw * nz;
-        T phi_r[PPT], rho_r[PPT];
-        cplx<T> cv[PPT];
-#pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-            phi_r[q] = T(0);
-            rho_r[q] = T(0);
-            cv[q] = gpe::mk(T(0), T(0));
-        }
-        // Phi of the tile rows: z C2R of G (Hermitian extension, kz = 0 zero)
-        if (mode & M_PINV) {
-            for (int e = threadIdx.x; e < rw * (nz + 1); e += NT) {
-                const int r = e / (nz + 1), kz = e - r * (nz + 1);
-                const cplx<T> v = kz ? gw[(size_t)(y0 + r) * nz + kz - 1] : gpe::mk(T(0), T(0));
-                X[r * zs + kz] = v;
-                if (kz >= 1 && kz < nz) X[r * zs + nz2 - kz] = gpe::mk(v.x, -v.y);
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.pz2, rw, 1, zs, +1, a.tw);
-            const T sc = (T)(a.pinv_scale * s_in * s_in);
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int p = threadIdx.x + q * NT;
-                if (p < np) {
-                    const int r = p / nz, z = p - r * nz;
-                    phi_r[q] = X[r * zs + z].x * sc;
-                    if (mode & M_PHI_STORE) a.phi[plane_off + (size_t)(y0 + r) * nz + z] = phi_r[q];
-                }
-            }
-            __syncthreads();
-        } else if (mode & M_PHI_LOAD) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int p = threadIdx.x + q * NT;
-                if (p < np) phi_r[q] = a.phi[plane_off + (size_t)y0 * nz + p];
-            }
-        }
-        // psi update
-        if (is_it) {
-            const cplx<T>* in = a.in;
-            const T sv = (T)s_in;
-            const T dtau = (T)a.dtau, cx = (T)a.cx, cy = (T)a.cy, cz = (T)a.cz, gq = (T)a.gq, gmq = (T)a.gmq,
-                    cdd = (T)a.cdd;
-            auto ld = [&](size_t e) {
-                const cplx<T> v = in[e];
-                return gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
-            };
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int p = threadIdx.x + q * NT;
-                if (p < np) {
-                    const int r = p / nz, k = p - r * nz, j = y0 + r;
-                    const size_t e = plane_off + (size_t)j * nz + k;
-                    cplx<T> o;
-                    if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
-                        const cplx<T> c = ld(e);
-                        const cplx<T> xm = in[e - npl], xp = in[e + npl];
-                        const cplx<T> ym = ld(e - nz), yp = ld(e + nz), zm = ld(e - 1), zp = ld(e + 1);
-                        const T v = a.V[e] + cdd * phi_r[q];
-                        cplx<T> c1;
-                        c1.x = cx * (xm.x * sv + xp.x * sv - T(2) * c.x) + cy * (ym.x + yp.x - T(2) * c.x) +
-                               cz * (zm.x + zp.x - T(2) * c.x) + c.x * v;
-                        c1.y = cx * (xm.y * sv + xp.y * sv - T(2) * c.y) + cy * (ym.y + yp.y - T(2) * c.y) +
-                               cz * (zm.y + zp.y - T(2) * c.y) + c.y * v;
-                        const T nrm = c.x * c.x + c.y * c.y;
-                        const T ab = sqrt(nrm);
-                        const T f = gq * nrm + gmq * (ab * ab * ab);
-                        o.x = c.x - dtau * c1.x;
-                        o.y = c.y - dtau * c1.y;
-                        o.x = o.x - dtau * (f * c.x);
-                        o.y = o.y - dtau * (f * c.y);
-                    } else {
-                        o = a.psii[e];
-                    }
-                    a.out[e] = o;
-                    cv[q] = o;
-                    rho_r[q] = o.x * o.x + o.y * o.y;
-                    nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
-                }
-            }
-        } else if (have_psi) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int p = threadIdx.x + q * NT;
-                if (p < np) {
-                    const int r = p / nz, k = p - r * nz, j = y0 + r;
-                    const size_t e = plane_off + (size_t)j * nz + k;
-                    cplx<T> c;
-                    if (mode & M_IN_TK) {
-                        c = a.tk[((size_t)j * nx + i) * nz + k];
-                    } else {
-                        c = a.in[e];
-                        if (mode & M_NORM_IN) c = gpe::mk((T)(c.x * s_in), (T)(c.y * s_in));
-                    }
-                    T nrm = c.x * c.x + c.y * c.y;
-                    if ((mode & (M_PHASE_HALF | M_PHASE_FULL)) && iint && j >= 1 && j <= ny - 2 && k >= 1 &&
-                        k <= nz - 2) {
-                        const double dn = (double)nrm;
-                        const double veff =
-                            a.cv * ((double)a.V[e] + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
-                        c = gpe::phase_rot(c, -a.hdt * veff);
-                        nrm = c.x * c.x + c.y * c.y;
-                    }
-                    cv[q] = c;
-                    rho_r[q] = nrm;
-                    nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
-                    if (mode & M_OUT) a.out[e] = c;
-                }
-            }
-        }
-        // z R2C of rho -> H
-        if (mode & M_RHO_FWD) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int p = threadIdx.x + q * NT;
-                if (p < np) {
-                    const int r = p / nz, z = p - r * nz;
-                    X[r * zs + z] = gpe::mk(rho_r[q], T(0));
-                    X[r * zs + nz + z] = gpe::mk(T(0), T(0));
-                }
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.pz2, rw, 1, zs, -1, a.tw);
-            for (int e = threadIdx.x; e < np; e += NT) {
-                const int r = e / nz, c = e - r * nz;
-                hs[(size_t)(y0 + r) * nz + c] = X[r * zs + c + 1];
-            }
-            __syncthreads();
-        }
-        // z kinetic of psi -> W (the G rows of this tile are consumed above)
-        if (mode & M_KIN) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int p = threadIdx.x + q * NT;
-                if (p < np) X[(p / nz) * qs + p % nz] = cv[q];
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.pz, rw, 1, qs, -1, a.tw, a.twlz);
-            for (int e = threadIdx.x; e < np; e += NT) {
-                const int qi = (e / nz) * qs + e % nz;
-                X[qi] = gpe::kin_mul(X[qi], a.ez[e % nz], 1.0 / nz);
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.pz, rw, 1, qs, +1, a.tw, a.twlz);
-            for (int e = threadIdx.x; e < np; e += NT) gw[(size_t)y0 * nz + e] = X[(e / nz) * qs + e % nz];
-            __syncthreads();
-        }
-    }
-
-    // ---- (C) column tiles: y2 of H -> S1o, y kinetic of W -> Tk
-    if (mode & M_RHO_FWD) {
-        const int cs = C + 1;
-        for (int c0 = 0; c0 < nz; c0 += C) {
-            const int cw = min(C, nz - c0);
-            for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
-                const int ky = e / cw, c = e - ky * cw;
-                X[ky * cs + c] = ky < ny ? hs[(size_t)ky * nz + c0 + c] : gpe::mk(T(0), T(0));
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.py2, cw, cs, 1, -1, a.tw);
-            for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
-                const int ky = e / cw, c = e - ky * cw;
-                a.s1o[((size_t)ky * nx + i) * nz + c0 + c] = X[ky * cs + c];
-            }
-            __syncthreads();
-        }
-    }
-    if (mode & M_KIN) {
-        const int cs = C + 1;
-        for (int c0 = 0; c0 < nz; c0 += C) {
-            const int cw = min(C, nz - c0);
-            for (int e = threadIdx.x; e < ny * cw; e += NT) {
-                const int y = e / cw, c = e - y * cw;
-                X[y * cs + c] = gw[(size_t)y * nz + c0 + c];
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.py, cw, cs, 1, -1, a.tw, a.twly);
-            for (int e = threadIdx.x; e < ny * cw; e += NT) {
-                const int y = e / cw, c = e - y * cw;
-                X[y * cs + c] = gpe::kin_mul(X[y * cs + c], a.ey[y], 1.0 / ny);
-            }
-            __syncthreads();
-            gpe::lds_fft<T, NT, MAXE>(X, a.py, cw, cs, 1, +1, a.tw, a.twly);
-            for (int e = threadIdx.x; e < ny * cw; e += NT) {
-                const int y = e / cw, c = e - y * cw;
-                a.tko[((size_t)y * nx + i) * nz + c0 + c] = X[y * cs + c];
-            }
-            __syncthreads();
-        }
-    }
-
-    if (mode & M_NORM_OUT) {
-        const double s = block_sum<NT>(nacc, red);
-        if (threadIdx.x == 0) a.part_out[i] = s;
     }
 }
 
-template <typename T, int NT>
-__global__ void __launch_bounds__(NT) k_plane_pencil(PlaneArgs<T> a) {
+// inverse y2 of the S1 columns c0 .. c0+cw (kz = c + 1) of plane i -> G
+template <typename T, int NT, class Pol>
+__global__ void __launch_bounds__(NT) k_pencil_ci(PlaneArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
+    const int nx = a.nx, ny = a.ny, nz = a.nz, ny2 = 2 * ny, C = Pol::C(a);
+    const int nct = (nz + C - 1) / C;
+    const int i = blockIdx.x / nct, c0 = (blockIdx.x - i * nct) * C;
+    const int cw = min(C, nz - c0), cs = C + 1;
+    cplx<T>* gw = a.pen_gw + (size_t)i * ny * nz;
+    for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
+        const int ky = e / cw, c = e - ky * cw;
+        X[ky * cs + c] = a.s1[((size_t)ky * nx + i) * nz + c0 + c];
+    }
+    __syncthreads();
+    Pol::template y2<T, NT, +1>(X, cw, cs, 1, a);
+    for (int e = threadIdx.x; e < ny * cw; e += NT) {
+        const int y = e / cw, c = e - y * cw;
+        gw[(size_t)y * nz + c0 + c] = X[y * cs + c];
+    }
+}
+
+// rows y0 .. y0+rw of plane i
+template <typename T, int NT, class Pol>
+__global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
+    constexpr int PPT = kPencilPPT;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[NT / 64 + 1];
-    pencil_body<T, NT>(a, blockIdx.x, a.mode, lds_raw, red);
+    cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
+    const int mode = a.mode;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int nz2 = 2 * nz, npl = ny * nz, R = Pol::R(a);
+    const int nrt = (ny + R - 1) / R;
+    const int i = blockIdx.x / nrt, tr = blockIdx.x - i * nrt, y0 = tr * R;
+    const int rw = min(R, ny - y0), np = rw * nz;
+    const size_t plane_off = (size_t)i * npl;
+    const bool is_it = (mode & M_IT) != 0;
+    const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
+    cplx<T>* gw = a.pen_gw + plane_off;  // [y][c]
+    cplx<T>* hs = a.pen_h + plane_off;   // [y][kz - 1]
+    double pg, pgm;
+    pencil_couplings(a, mode, &pg, &pgm);
+    double s_in = 1.0;
+    if (mode & M_NORM_IN) s_in = 1.0 / sqrt(sum_partials(a.part_in, nx, red));
+    const int zs = nz2 + 1, qs = nz + 1;
+    const bool iint = (i >= 1 && i <= nx - 2);
+    double nacc = 0.0;
+    T phi_r[PPT], rho_r[PPT];
+    cplx<T> cv[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        phi_r[q] = T(0);
+        rho_r[q] = T(0);
+        cv[q] = gpe::mk(T(0), T(0));
+    }
+    // Phi of the tile rows: z C2R of G (Hermitian extension, kz = 0 zero)
+    if (mode & M_PINV) {
+        for (int e = threadIdx.x; e < rw * (nz + 1); e += NT) {
+            const int r = e / (nz + 1), kz = e - r * (nz + 1);
+            const cplx<T> v = kz ? gw[(size_t)(y0 + r) * nz + kz - 1] : gpe::mk(T(0), T(0));
+            X[r * zs + kz] = v;
+            if (kz >= 1 && kz < nz) X[r * zs + nz2 - kz] = gpe::mk(v.x, -v.y);
+        }
+        __syncthreads();
+        Pol::template z2<T, NT, +1>(X, rw, 1, zs, a);
+        const T sc = (T)(a.pinv_scale * s_in * s_in);
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < np) {
+                const int r = p / nz, z = p - r * nz;
+                phi_r[q] = X[r * zs + z].x * sc;
+                if (mode & M_PHI_STORE) a.phi[plane_off + (size_t)(y0 + r) * nz + z] = phi_r[q];
+            }
+        }
+        __syncthreads();
+    } else if (mode & M_PHI_LOAD) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < np) phi_r[q] = a.phi[plane_off + (size_t)y0 * nz + p];
+        }
+    }
+    // psi update
+    if (is_it) {
+        const cplx<T>* in = a.in;
+        const T sv = (T)s_in;
+        const T dtau = (T)a.dtau, cx = (T)a.cx, cy = (T)a.cy, cz = (T)a.cz, gq = (T)a.gq, gmq = (T)a.gmq,
+                cdd = (T)a.cdd;
+        auto ld = [&](size_t e) {
+            const cplx<T> v = in[e];
+            return gpe::mk((T)(v.x * s_in), (T)(v.y * s_in));
+        };
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < np) {
+                const int r = p / nz, k = p - r * nz, j = y0 + r;
+                const size_t e = plane_off + (size_t)j * nz + k;
+                cplx<T> o;
+                if (iint && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
+                    const cplx<T> c = ld(e);
+                    const cplx<T> xm = in[e - npl], xp = in[e + npl];
+                    const cplx<T> ym = ld(e - nz), yp = ld(e + nz), zm = ld(e - 1), zp = ld(e + 1);
+                    const T v = a.V[e] + cdd * phi_r[q];
+                    cplx<T> c1;
+                    c1.x = cx * (xm.x * sv + xp.x * sv - T(2) * c.x) + cy * (ym.x + yp.x - T(2) * c.x) +
+                           cz * (zm.x + zp.x - T(2) * c.x) + c.x * v;
+                    c1.y = cx * (xm.y * sv + xp.y * sv - T(2) * c.y) + cy * (ym.y + yp.y - T(2) * c.y) +
+                           cz * (zm.y + zp.y - T(2) * c.y) + c.y * v;
+                    const T nrm = c.x * c.x + c.y * c.y;
+                    const T ab = sqrt(nrm);
+                    const T f = gq * nrm + gmq * (ab * ab * ab);
+                    o.x = c.x - dtau * c1.x;
+                    o.y = c.y - dtau * c1.y;
+                    o.x = o.x - dtau * (f * c.x);
+                    o.y = o.y - dtau * (f * c.y);
+                } else {
+                    o = a.psii[e];
+                }
+                a.out[e] = o;
+                cv[q] = o;
+                rho_r[q] = o.x * o.x + o.y * o.y;
+                nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+            }
+        }
+    } else if (have_psi) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < np) {
+                const int r = p / nz, k = p - r * nz, j = y0 + r;
+                const size_t e = plane_off + (size_t)j * nz + k;
+                cplx<T> c;
+                if (mode & M_IN_TK) {
+                    c = a.tk[((size_t)j * nx + i) * nz + k];
+                } else {
+                    c = a.in[e];
+                    if (mode & M_NORM_IN) c = gpe::mk((T)(c.x * s_in), (T)(c.y * s_in));
+                }
+                T nrm = c.x * c.x + c.y * c.y;
+                if ((mode & (M_PHASE_HALF | M_PHASE_FULL)) && iint && j >= 1 && j <= ny - 2 && k >= 1 &&
+                    k <= nz - 2) {
+                    const double dn = (double)nrm;
+                    const double veff =
+                        a.cv * ((double)a.V[e] + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
+                    c = gpe::phase_rot(c, -a.hdt * veff);
+                    nrm = c.x * c.x + c.y * c.y;
+                }
+                cv[q] = c;
+                rho_r[q] = nrm;
+                nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
+                if (mode & M_OUT) a.out[e] = c;
+            }
+        }
+    }
+    // z R2C of rho -> H
+    if (mode & M_RHO_FWD) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < np) {
+                const int r = p / nz, z = p - r * nz;
+                X[r * zs + z] = gpe::mk(rho_r[q], T(0));
+                X[r * zs + nz + z] = gpe::mk(T(0), T(0));
+            }
+        }
+        __syncthreads();
+        Pol::template z2<T, NT, -1>(X, rw, 1, zs, a);
+        for (int e = threadIdx.x; e < np; e += NT) {
+            const int r = e / nz, c = e - r * nz;
+            hs[(size_t)(y0 + r) * nz + c] = X[r * zs + c + 1];
+        }
+        __syncthreads();
+    }
+    // z kinetic of psi -> W (the G rows of this tile are consumed above)
+    if (mode & M_KIN) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < np) X[(p / nz) * qs + p % nz] = cv[q];
+        }
+        __syncthreads();
+        Pol::template z<T, NT, -1>(X, rw, 1, qs, a);
+        for (int e = threadIdx.x; e < np; e += NT) {
+            const int qi = (e / nz) * qs + e % nz;
+            X[qi] = gpe::kin_mul(X[qi], a.ez[e % nz], 1.0 / nz);
+        }
+        __syncthreads();
+        Pol::template z<T, NT, +1>(X, rw, 1, qs, a);
+        for (int e = threadIdx.x; e < np; e += NT) gw[(size_t)y0 * nz + e] = X[(e / nz) * qs + e % nz];
+    }
+    if (mode & M_NORM_OUT) {
+        const double sum = block_sum<NT>(nacc, red);
+        if (threadIdx.x == 0) a.pen_part[(size_t)i * nrt + tr] = sum;
+    }
+}
+
+// blocks [0, nx*nct): y2 of H -> S1o (M_RHO_FWD); [nx*nct, 2 nx*nct): y
+// kinetic of W -> Tk (M_KIN); the first block of each plane also sums the
+// plane's tile norm partials in order (M_NORM_OUT)
+template <typename T, int NT, class Pol>
+__global__ void __launch_bounds__(NT) k_pencil_cf(PlaneArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* X = reinterpret_cast<cplx<T>*>(lds_raw);
+    const int mode = a.mode;
+    const int nx = a.nx, ny = a.ny, nz = a.nz, ny2 = 2 * ny, C = Pol::C(a);
+    const int nct = (nz + C - 1) / C;
+    int b = blockIdx.x;
+    const bool kin = b >= nx * nct;
+    if (kin) b -= nx * nct;
+    const int i = b / nct, c0 = (b - i * nct) * C;
+    const int cw = min(C, nz - c0), cs = C + 1;
+    const size_t plane_off = (size_t)i * ny * nz;
+    if (!kin && c0 == 0 && (mode & M_NORM_OUT) && threadIdx.x == 0) {
+        const int nrt = (ny + Pol::R(a) - 1) / Pol::R(a);
+        double sum = 0.0;
+        for (int t = 0; t < nrt; ++t) sum += a.pen_part[(size_t)i * nrt + t];
+        a.part_out[i] = sum;
+    }
+    if (!kin && (mode & M_RHO_FWD)) {
+        const cplx<T>* hs = a.pen_h + plane_off;
+        for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
+            const int ky = e / cw, c = e - ky * cw;
+            X[ky * cs + c] = ky < ny ? hs[(size_t)ky * nz + c0 + c] : gpe::mk(T(0), T(0));
+        }
+        __syncthreads();
+        Pol::template y2<T, NT, -1>(X, cw, cs, 1, a);
+        for (int e = threadIdx.x; e < ny2 * cw; e += NT) {
+            const int ky = e / cw, c = e - ky * cw;
+            a.s1o[((size_t)ky * nx + i) * nz + c0 + c] = X[ky * cs + c];
+        }
+    }
+    if (kin && (mode & M_KIN)) {
+        const cplx<T>* gw = a.pen_gw + plane_off;
+        for (int e = threadIdx.x; e < ny * cw; e += NT) {
+            const int y = e / cw, c = e - y * cw;
+            X[y * cs + c] = gw[(size_t)y * nz + c0 + c];
+        }
+        __syncthreads();
+        Pol::template y<T, NT, -1>(X, cw, cs, 1, a);
+        for (int e = threadIdx.x; e < ny * cw; e += NT) {
+            const int y = e / cw, c = e - y * cw;
+            X[y * cs + c] = gpe::kin_mul(X[y * cs + c], a.ey[y], 1.0 / ny);
+        }
+        __syncthreads();
+        Pol::template y<T, NT, +1>(X, cw, cs, 1, a);
+        for (int e = threadIdx.x; e < ny * cw; e += NT) {
+            const int y = e / cw, c = e - y * cw;
+            a.tko[((size_t)y * nx + i) * nz + c0 + c] = X[y * cs + c];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1826,6 +1894,7 @@ struct gpe_handle {
     bool pencil = false;        // plane beyond one block's LDS: the pencil plane pass
     void* pen_gw = nullptr;     // its per-plane intermediates (N complex each)
     void* pen_h = nullptr;
+    double* pen_part = nullptr;  // norm partial per (plane, row tile)
     int pen_c = 0, pen_r = 0;   // column / row tile shapes
     size_t pen_lds = 0;
     LinePlan px, py, pz, px2, py2, pz2;
@@ -2074,6 +2143,7 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.pen_h = (cplx<T>*)h->pen_h;
     a.pen_c = h->pen_c;
     a.pen_r = h->pen_r;
+    a.pen_part = h->pen_part;
     a.pz2 = h->pz2;
     a.py2 = h->py2;
     a.pz = h->pz;
@@ -2161,16 +2231,42 @@ static int launch_plane_pol(gpe_handle* h, const PlaneArgs<T>& a) {
 }
 
 
+// The pencil plane pass: three launches; the timing events bracket all three.
+template <typename T, class Pol>
+static int launch_pencil(gpe_handle* h, const PlaneArgs<T>& a) {
+    hipEvent_t e0, e1;
+    kernel_events(h, 0, &e0, &e1);
+    const int nx = h->p.nx, ny = h->p.ny, nz = h->p.nz;
+    const int nct = (nz + h->pen_c - 1) / h->pen_c, nrt = (ny + h->pen_r - 1) / h->pen_r;
+    const bool ci = (a.mode & M_PINV) != 0;
+    const bool cf = (a.mode & (M_RHO_FWD | M_KIN | M_NORM_OUT)) != 0;
+    if (ci) {
+        hipExtLaunchKernelGGL((k_pencil_ci<T, kPencilThreads, Pol>), dim3(nx * nct), dim3(kPencilThreads), h->pen_lds,
+                              h->stream, e0, nullptr, 0, a);
+        GPE_CHECK(hipGetLastError());
+    }
+    hipExtLaunchKernelGGL((k_pencil_r<T, kPencilThreads, Pol>), dim3(nx * nrt), dim3(kPencilThreads), h->pen_lds,
+                          h->stream, ci ? nullptr : e0, cf ? nullptr : e1, 0, a);
+    GPE_CHECK(hipGetLastError());
+    if (cf) {
+        hipExtLaunchKernelGGL((k_pencil_cf<T, kPencilThreads, Pol>), dim3(2 * nx * nct), dim3(kPencilThreads),
+                              h->pen_lds, h->stream, nullptr, e1, 0, a);
+        GPE_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
 template <typename T>
 static int launch_plane(gpe_handle* h, const PlaneArgs<T>& a) {
     const int ny = h->p.ny, nz = h->p.nz;
     if (h->pencil) {
-        hipEvent_t e0, e1;
-        kernel_events(h, 0, &e0, &e1);
-        hipExtLaunchKernelGGL((k_plane_pencil<T, kPencilThreads>), dim3(h->p.nx), dim3(kPencilThreads), h->pen_lds,
-                              h->stream, e0, e1, 0, a);
-        GPE_CHECK(hipGetLastError());
-        return 0;
+        if (!h->force_generic) {
+#define GPE_PP(NY, NZ, CC, RR) \
+    if (ny == NY && nz == NZ) return launch_pencil<T, PencilPolCT<NY, NZ, CC, RR>>(h, a);
+            GPE_PENCIL_SHAPES(GPE_PP)
+#undef GPE_PP
+        }
+        return launch_pencil<T, PencilPolGeneric>(h, a);
     }
     if (!h->force_generic) {
 #define GPE_PL(NY, NZ) \
@@ -2907,8 +3003,21 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
             h->pen_c = std::min(h->pen_c, tc);
             h->pen_r = std::min(h->pen_r, tr);
         }
+        // compile-time pencil shapes use their own tiles
+        if (!h->force_generic) {
+#define GPE_PP(NY, NZ, CC, RR)          \
+    if (p->ny == NY && p->nz == NZ) { \
+        h->pen_c = CC;                  \
+        h->pen_r = RR;                  \
+    }
+            GPE_PENCIL_SHAPES(GPE_PP)
+#undef GPE_PP
+        }
         h->pen_lds = (size_t)std::max(2L * p->ny * (h->pen_c + 1), (long)h->pen_r * (2 * p->nz + 1)) * cs;
-        if (hipMalloc(&h->pen_gw, n * cs) || hipMalloc(&h->pen_h, n * cs)) return bad("device allocation failed");
+        const size_t nrt = (p->ny + h->pen_r - 1) / h->pen_r;
+        if (hipMalloc(&h->pen_gw, n * cs) || hipMalloc(&h->pen_h, n * cs) ||
+            hipMalloc(&h->pen_part, (size_t)p->nx * nrt * sizeof(double)))
+            return bad("device allocation failed");
     }
     if (hipDeviceSynchronize() != hipSuccess) return bad("device synchronize failed after setup");
     *out = h;
@@ -2920,7 +3029,7 @@ void gpe_destroy(gpe_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->store[0], h->store[1], h->psii,     h->V,       h->phi,    h->s1,  h->tk,
                     h->s1b,      h->tkb,      h->tw,       h->twl,     h->ex,     h->ey,  h->ez,  h->kx2,
-                    h->ky2,      h->kz2,      h->part[0],  h->part[1], h->epart,  h->pen_gw, h->pen_h};
+                    h->ky2,      h->kz2,      h->part[0],  h->part[1], h->epart,  h->pen_gw, h->pen_h, h->pen_part};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
