@@ -1656,43 +1656,41 @@ __global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_k) {
         xline_reg<T, X::LP, X::LPB_P, true>(a, blockIdx.x - nblk_k, E, TB);
 }
 
-// Batched launches: one block per work item (instance, plane) / (instance,
-// tile); each instance brings its own argument block and may have its own grid.
-template <typename T, int NT>
-__global__ void __launch_bounds__(NT) k_plane_mixed(const PlaneArgs<T>* __restrict__ args,
+// Batched launches, one per grid shape: one block per work item (instance,
+// plane) / (instance, tile); each instance brings its own argument block.
+// A launch per shape keeps the single-instance kernels' register budget (the
+// compiled FP32 plane pass runs 1024 threads in 128 VGPRs; one kernel
+// holding every shape would not).
+template <typename T, int NT, class Pol>
+__global__ void __launch_bounds__(NT) k_plane_group(const PlaneArgs<T>* __restrict__ args,
                                                     const int2* __restrict__ work) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[NT / 64 + 1];
     const int2 w = work[blockIdx.x];
     const PlaneArgs<T> a = args[w.x];
-    if (!a.generic) {
-#define GPE_PL(NY, NZ)                                                  \
-    if (a.ny == NY && a.nz == NZ) {                                     \
-        plane_body<T, NT, PlanePolCT<NY, NZ>>(a, w.y, a.mode, lds_raw, red); \
-        return;                                                         \
-    }
-        GPE_PLANE_SHAPES(GPE_PL)
-#undef GPE_PL
-    }
-    plane_body<T, NT, PlanePolGeneric>(a, w.y, a.mode, lds_raw, red);
+    plane_body<T, NT, Pol>(a, w.y, a.mode, lds_raw, red);
 }
 
-template <typename T, int NT>
-__global__ void __launch_bounds__(NT) k_xpass_mixed(const LineArgs<T>* __restrict__ args,
+template <typename T, int NT, class Pol>
+__global__ void __launch_bounds__(NT) k_xpass_group(const LineArgs<T>* __restrict__ args,
                                                     const int2* __restrict__ work) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int2 w = work[blockIdx.x];
     const LineArgs<T> a = args[w.x];
-    if (!a.generic) {
-#define GPE_LN(NX)                                              \
-    if (a.nx == NX) {                                           \
-        line_body<T, NT, LinePolCT<NX>>(a, w.y, lds_raw);       \
-        return;                                                 \
-    }
-        GPE_LINE_SHAPES(GPE_LN)
-#undef GPE_LN
-    }
-    line_body<T, NT, LinePolGeneric>(a, w.y, lds_raw);
+    line_body<T, NT, Pol>(a, w.y, lds_raw);
+}
+
+// Register x pass, Poisson tiles only (imaginary time).
+template <typename T, int NX>
+__global__ void __launch_bounds__(256) k_xpass_reg_group(const LineArgs<T>* __restrict__ args,
+                                                         const int2* __restrict__ work) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
+    cplx<T>* TB = E + 16 * 256 + 256;
+    using X = XReg<NX>;
+    const int2 w = work[blockIdx.x];
+    const LineArgs<T> a = args[w.x];
+    xline_reg<T, X::LP, X::LPB_P, true>(a, w.y, E, TB);
 }
 
 // ---------------------------------------------------------------------------
@@ -2709,7 +2707,13 @@ struct gpe_batch {
     void* d_pwork = nullptr;
     void* d_lwork = nullptr;
     int npwork = 0, nlwork = 0;
-    size_t plane_lds = 0, line_lds = 0;
+    // one launch per grid shape: plane groups by (ny, nz), line groups by nx
+    // (kind 1: register x pass); work items of a group are contiguous
+    struct Group {
+        int a, b, kind, off, cnt;
+        size_t lds;
+    };
+    std::vector<Group> pgroups, lgroups;
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     bool use_graphs = true;
 };
@@ -2744,7 +2748,8 @@ static int batch_setup(gpe_batch* b) {
     std::vector<PlaneArgs<T>> pa(12 * na);
     std::vector<LineArgs<T>> la(na);
     std::vector<int2> pw, lw;
-    b->plane_lds = b->line_lds = 0;
+    b->pgroups.clear();
+    b->lgroups.clear();
     for (size_t q = 0; q < na; ++q) {
         gpe_handle* h = b->inst[b->active[q]];
         for (int par = 0; par < 4; ++par) {
@@ -2765,10 +2770,49 @@ static int batch_setup(gpe_batch* b) {
             }
         }
         la[q] = line_args<T>(h, true, false);
-        for (int i = 0; i < h->p.nx; ++i) pw.push_back(make_int2((int)q, i));
-        for (int t = 0; t < la[q].ntile_p; ++t) lw.push_back(make_int2((int)q, t));
-        b->plane_lds = std::max(b->plane_lds, h->plane_lds + (plane_compiled(h) ? h->plane_stage : 0));
-        b->line_lds = std::max(b->line_lds, h->line_lds);
+    }
+    // work lists grouped by shape, instances in order within a group
+    const bool gen = !b->inst.empty() && b->inst[0]->force_generic;
+    for (size_t q = 0; q < na; ++q) {
+        gpe_handle* h = b->inst[b->active[q]];
+        bool seen = false;
+        for (auto& g : b->pgroups) seen = seen || (g.a == h->p.ny && g.b == h->p.nz);
+        if (seen) continue;
+        {
+            gpe_batch::Group g{h->p.ny, h->p.nz, 0, (int)pw.size(), 0, 0};
+            for (size_t r = q; r < na; ++r) {
+                gpe_handle* hr = b->inst[b->active[r]];
+                if (hr->p.ny != g.a || hr->p.nz != g.b) continue;
+                for (int i = 0; i < hr->p.nx; ++i) pw.push_back(make_int2((int)r, i));
+                g.lds = std::max(g.lds, hr->plane_lds + (plane_compiled(hr) ? hr->plane_stage : 0));
+            }
+            g.cnt = (int)pw.size() - g.off;
+            b->pgroups.push_back(g);
+        }
+    }
+    for (size_t q = 0; q < na; ++q) {
+        gpe_handle* h = b->inst[b->active[q]];
+        bool seen = false;
+        for (auto& g : b->lgroups) seen = seen || g.a == h->p.nx;
+        if (seen) continue;
+        const int nx = h->p.nx;
+        const bool reg = !gen && h->line_reg && (nx == 128 || nx == 64);
+        gpe_batch::Group g{nx, 0, reg ? 1 : 0, (int)lw.size(), 0, 0};
+        for (size_t r = q; r < na; ++r) {
+            gpe_handle* hr = b->inst[b->active[r]];
+            if (hr->p.nx != nx) continue;
+            int nt = la[r].ntile_p;
+            if (reg) {
+                const int lpb = nx == 128 ? XReg<128>::LPB_P : XReg<64>::LPB_P;
+                nt = (2 * hr->p.ny * hr->p.nz + lpb - 1) / lpb;
+                g.lds = (size_t)(16 * 256 + 256 + 4 * nx) * sizeof(cplx<T>);
+            } else {
+                g.lds = std::max(g.lds, hr->line_lds);
+            }
+            for (int t = 0; t < nt; ++t) lw.push_back(make_int2((int)r, t));
+        }
+        g.cnt = (int)lw.size() - g.off;
+        b->lgroups.push_back(g);
     }
     void* old[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork};
     for (void* o : old)
@@ -2792,18 +2836,60 @@ static int batch_setup(gpe_batch* b) {
 template <typename T>
 static int batch_plane(gpe_batch* b, int slot) {
     const PlaneArgs<T>* args = (const PlaneArgs<T>*)b->d_pargs + (size_t)slot * b->active.size();
-    constexpr int NT = PlaneThreads<T, false>::value;
-    hipLaunchKernelGGL((k_plane_mixed<T, NT>), dim3(b->npwork), dim3(NT), b->plane_lds,
-                       b->stream, args, (const int2*)b->d_pwork);
-    BCHECK(hipGetLastError());
+    const bool gen = b->inst[0]->force_generic;
+    for (const auto& g : b->pgroups) {
+        const int2* w = (const int2*)b->d_pwork + g.off;
+        bool done = false;
+        if (!gen) {
+#define GPE_PL(NY, NZ)                                                                                      \
+    if (!done && g.a == NY && g.b == NZ) {                                                                  \
+        constexpr int NT = PlaneThreads<T, true>::value;                                                    \
+        hipLaunchKernelGGL((k_plane_group<T, NT, PlanePolCT<NY, NZ>>), dim3(g.cnt), dim3(NT), g.lds, b->stream, \
+                           args, w);                                                                        \
+        done = true;                                                                                        \
+    }
+            GPE_PLANE_SHAPES(GPE_PL)
+#undef GPE_PL
+        }
+        if (!done) {
+            constexpr int NT = PlaneThreads<T, false>::value;
+            hipLaunchKernelGGL((k_plane_group<T, NT, PlanePolGeneric>), dim3(g.cnt), dim3(NT), g.lds, b->stream, args,
+                               w);
+        }
+        BCHECK(hipGetLastError());
+    }
     return 0;
 }
 
 template <typename T>
 static int batch_line(gpe_batch* b) {
-    hipLaunchKernelGGL((k_xpass_mixed<T, kLineThreads>), dim3(b->nlwork), dim3(kLineThreads), b->line_lds,
-                       b->stream, (const LineArgs<T>*)b->d_largs, (const int2*)b->d_lwork);
-    BCHECK(hipGetLastError());
+    const LineArgs<T>* args = (const LineArgs<T>*)b->d_largs;
+    const bool gen = b->inst[0]->force_generic;
+    for (const auto& g : b->lgroups) {
+        const int2* w = (const int2*)b->d_lwork + g.off;
+        bool done = false;
+        if (g.kind == 1 && g.a == 128) {
+            hipLaunchKernelGGL((k_xpass_reg_group<T, 128>), dim3(g.cnt), dim3(256), g.lds, b->stream, args, w);
+            done = true;
+        } else if (g.kind == 1 && g.a == 64) {
+            hipLaunchKernelGGL((k_xpass_reg_group<T, 64>), dim3(g.cnt), dim3(256), g.lds, b->stream, args, w);
+            done = true;
+        }
+        if (!done && !gen) {
+#define GPE_LN(NX)                                                                                           \
+    if (!done && g.a == NX) {                                                                                \
+        hipLaunchKernelGGL((k_xpass_group<T, kLineThreads, LinePolCT<NX>>), dim3(g.cnt), dim3(kLineThreads), \
+                           g.lds, b->stream, args, w);                                                       \
+        done = true;                                                                                         \
+    }
+            GPE_LINE_SHAPES(GPE_LN)
+#undef GPE_LN
+        }
+        if (!done)
+            hipLaunchKernelGGL((k_xpass_group<T, kLineThreads, LinePolGeneric>), dim3(g.cnt), dim3(kLineThreads),
+                               g.lds, b->stream, args, w);
+        BCHECK(hipGetLastError());
+    }
     return 0;
 }
 

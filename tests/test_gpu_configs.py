@@ -124,3 +124,56 @@ def test_first_steps_against_oracle_128x128x32(oracle):
         assert r < (1e-10 if prec == hip.GPE_FP64 else 2e-5)
         assert abs(e["e_total"] - eo_rt["e_total"]) < tol * abs(eo_rt["e_total"])
         dev.close()
+
+
+def sweep_instance(rng, shape):
+    """One instance of the batched IT sweep: edd ~ U[1.2, 1.6], n_atoms
+    log-uniform in [5e3, 5e4], omega_z ~ U[100, 140] Hz; MEXICAN with
+    dd = 1500 nm and MULTIPLE_GAUSS(2) (4-cell widths) plus 1e-3 rad noise."""
+    nx, ny, nz = shape
+    edd = rng.uniform(1.2, 1.6)
+    n_atoms = 10 ** rng.uniform(np.log10(5e3), np.log10(5e4))
+    wz = rng.uniform(100.0, 140.0)
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, *D, n_atoms=n_atoms, edd=edd, omega_hz=(60.0, 60.0, wz),
+                                           dd_nm=1500.0)
+    V = P.potential("MEXICAN", p)
+    psi = P.add_phase_noise(P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2, sigma_cells=4.0), 1e-3,
+                            int(rng.integers(1 << 30)))
+    return p, V, psi
+
+
+def test_config3_batched_it_sweep():
+    """64 instances over 64x64x16, 96x96x24 and 128x128x32 (evenly mixed), FP32
+    bulk, one launch per pass over every active instance, each to
+    |dE/N| < 1e-6 between 100-step checks or 5000 steps."""
+    import time
+
+    rng = np.random.default_rng(20261203)
+    shapes = [(64, 64, 16), (96, 96, 24), (128, 128, 32)]
+    setups = [sweep_instance(rng, shapes[q % 3]) for q in range(64)]
+    b = hip.Batch([s[0].to_gpe() for s in setups], hip.GPE_FP32)
+    for d, (p, V, psi) in zip(b.instances, setups):
+        d.load_pote(V)
+        d.load_buffer(psi)
+    t0 = time.perf_counter()
+    steps, e = b.it_until(5000, check_every=100, tol=1e-6)
+    wall = time.perf_counter() - t0
+    conv = steps < 5000
+    pts = np.array([s[0].nx * s[0].ny * s[0].nz for s in setups], dtype=np.float64)
+    print(f"\nconfig 3 sweep: {wall:.2f} s, {conv.sum()} of 64 converged, steps mean {steps.mean():.0f} "
+          f"max {steps.max()}, {np.sum(steps * pts) / wall / 1e9:.2f} G point-steps/s", flush=True)
+    assert np.all(np.isfinite(e)) and np.all(steps % 100 == 0) and np.all(steps <= 5000)
+    assert conv.sum() >= 32
+    # one instance of each grid again on its own handle: the same stop and E/N
+    for q in range(3):
+        p, V, psi = setups[q]
+        d = hip.Device(p.to_gpe(), hip.GPE_FP32)
+        d.load_pote(V)
+        d.load_buffer(psi)
+        s1, e1 = d.it_until(5000, check_every=100, tol=1e-6)
+        print(f"  instance {q} {shapes[q]}: batch {int(steps[q])} steps E/N {e[q]:.9e}, single {s1} steps "
+              f"E/N {e1:.9e}", flush=True)
+        assert abs(s1 - int(steps[q])) <= 200
+        assert abs(e1 - e[q]) <= 1e-5 * abs(e1)
+        d.close()
+    b.close()
