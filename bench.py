@@ -20,7 +20,11 @@ Prints ONE JSON line.  Besides steps/s it reports:
                 against the same peak;
   cpu_baseline  the FP64 CPU oracle (the project's restatement of
                 cpu_solver + FFTW solvers) timed on this host's cores on a
-                bounded sample of the same workload.
+                bounded sample of the same workload;
+  it_steps_per_s, it_time_to_converge
+                fused imaginary-time throughput on the same grid, and the
+                wall time of plain IT from MULTIPLE_GAUSS to |dE/N| < 1e-6
+                between 100-step checks (energies included).
 """
 from __future__ import annotations
 
@@ -50,7 +54,7 @@ def measured_traffic(kernel: str):
         return None
 
 
-def build_case(precision):
+def build_case(precision, sigma_cells=0.0):
     from gpesolver_amd import hip
     from gpesolver_amd import params as P
 
@@ -59,7 +63,8 @@ def build_case(precision):
                                            omega_hz=(60.0, 60.0, 120.0), dd_nm=1500.0)
     p.real_time_dt = 1e-4 / p.omega_x
     V = P.potential("MEXICAN", p)
-    psi = P.add_phase_noise(P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2), 1e-3, 20261203)
+    psi = P.add_phase_noise(P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2, sigma_cells=sigma_cells), 1e-3,
+                            20261203)
     dev = hip.Device(p.to_gpe(), precision)
     dev.load_pote(V)
     dev.load_buffer(psi)
@@ -140,6 +145,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64", action="store_true", help="also time the FP64 path")
+    ap.add_argument("--it-max-steps", type=int, default=20000, help="cap of the IT time-to-convergence leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="end with one device-to-device copy of this many MiB (the known-size "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -184,6 +190,16 @@ def main():
     it_el = time.perf_counter() - t1
     e = dev.energy()
     dev.close()
+
+    # IT time to |dE/N| < 1e-6 between 100-step checks from the plain
+    # MULTIPLE_GAUSS start (4-cell widths), FP32, this grid
+    pc, dc, _, _ = build_case(hip.GPE_FP32, sigma_cells=4.0)
+    dc.autotune(200)
+    dc.synchronize()
+    t1 = time.perf_counter()
+    it_conv_steps, it_conv_e = dc.it_until(args.it_max_steps, check_every=100, tol=1e-6)
+    it_conv_s = time.perf_counter() - t1
+    dc.close()
 
     extra = {}
     if args.fp64:
@@ -242,6 +258,10 @@ def main():
         },
         "it_steps_per_s": nit / it_el,
         "it_energy_per_atom": e["e_total"] / p.n_atoms,
+        "it_time_to_converge": {"tol_dE_per_N": 1e-6, "check_every": 100, "steps": it_conv_steps,
+                                "seconds": it_conv_s, "energy_per_atom": it_conv_e,
+                                "converged": it_conv_steps < args.it_max_steps,
+                                "start": "MULTIPLE_GAUSS(2), 4-cell widths, 1e-3 rad phase noise"},
         **extra,
     }
     if not args.no_cpu:
