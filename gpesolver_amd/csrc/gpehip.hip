@@ -64,9 +64,9 @@ __device__ __forceinline__ void store_nt(cplx<T>* p, cplx<T> v) {
 namespace {
 
 // Threads of a plane block.  The compiled FP32 shapes run 16 waves per
-// plane (more loads in flight per CU, and they fit in 128 VGPRs); FP64, the
-// run-time-size kernel and the batched kernel (which holds every shape) run
-// 8 waves, since they need more than 128 VGPRs.
+// plane (more loads in flight per CU, and they fit in 128 VGPRs); FP64 and
+// the run-time-size kernel run 8 waves, since they need more than 128 VGPRs.
+// The batch launches the same kernels per grid shape.
 template <typename T, bool COMPILED>
 struct PlaneThreads {
     static constexpr int value = (sizeof(T) == 4 && COMPILED) ? 1024 : 512;
