@@ -104,3 +104,30 @@ def test_single_instance_until_matches_batch_of_one(prec):
     assert steps2 == 250
     d.close()
     b.close()
+
+
+@pytest.mark.parametrize("generic", ["0", "1"])
+def test_batch_groups_share_shapes_and_generic_kernels(monkeypatch, generic):
+    """Instances sharing a plane shape but not nx (and the reverse) fall into
+    the same launch group; with GPE_FORCE_GENERIC=1 every group runs the
+    run-time-size kernels.  Each batched instance equals its single handle
+    bit for bit (FP64)."""
+    monkeypatch.setenv("GPE_FORCE_GENERIC", generic)
+    cases = [((32, 24, 12), 1.5, 2e4), ((24, 24, 12), 1.4, 1e4), ((32, 16, 8), 1.3, 8e3), ((64, 16, 8), 1.45, 3e4)]
+    setups = [setup_instance(*c) for c in cases]
+    b = hip.Batch([s[0].to_gpe() for s in setups], hip.GPE_FP64)
+    for d, (p, V, psi) in zip(b.instances, setups):
+        d.load_pote(V)
+        d.load_buffer(psi)
+    steps, e = b.it_until(60, check_every=30, tol=0.0)
+    assert list(steps) == [60] * len(cases)
+    for q, (p, V, psi) in enumerate(setups):
+        s = hip.Device(p.to_gpe(), hip.GPE_FP64)
+        s.load_pote(V)
+        s.load_buffer(psi)
+        s.it_steps(30)
+        s.energy()
+        s.it_steps(30)
+        assert np.array_equal(b.instances[q].export_psi(), s.export_psi()), q
+        s.close()
+    b.close()
