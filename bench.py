@@ -138,6 +138,50 @@ def cpu_baseline(threads: int, budget_s: float = 12.0):
             "sample": f"{n} FP64 RT steps of the same 128x128x32 workload ({el:.1f} s), FFT threaded with OpenMP"}
 
 
+def large_rt(steps: int, warmup: int, fp64: bool) -> dict:
+    """256x256x64 (dx = dy = 60 nm, dz = 300 nm), 1e5 atoms, MEXICAN_ASYMETRIC,
+    SETUP_GAUSS 3x3x2, eps_dd ramped 1.5 -> 1.3 over 20000 steps: RT steps/s
+    of the pencil plane pass, FP32 (and FP64 with --fp64)."""
+    from gpesolver_amd import hip
+    from gpesolver_amd import params as P
+
+    nx, ny, nz = 256, 256, 64
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, 60.0, 60.0, 300.0, n_atoms=1e5, edd=None, edd_start=1.5,
+                                           edd_stop=1.3, iter_real=20000)
+    V = P.potential("MEXICAN_ASYMETRIC", p)
+    psi = P.wavefunction("SETUP_GAUSS", p, drops=(3, 3, 2))
+    n = warmup + steps
+    g = np.array([p.coupling_at(t)[1] for t in range(n)])
+    gm = np.array([p.coupling_at(t)[2] for t in range(n)])
+    out = {"metric": "RT split-step steps/s @256x256x64 (pencil plane pass)", "unit": "steps/s", "steps": steps,
+           "warmup": warmup, "config": {"workload": "RT 256x256x64, MEXICAN_ASYMETRIC, SETUP_GAUSS 3x3x2, "
+                                                    "edd ramp 1.5->1.3 over 20000 steps"}}
+    for prec, name in ((hip.GPE_FP32, "fp32"), (hip.GPE_FP64, "fp64")):
+        if prec == hip.GPE_FP64 and not fp64:
+            continue
+        d = hip.Device(p.to_gpe(), prec)
+        d.autotune(50)
+        d.load_pote(V)
+        d.load_buffer(psi)
+        d.rt_steps(warmup, g[:warmup], gm[:warmup])
+        d.synchronize()
+        t0 = time.perf_counter()
+        d.rt_steps(steps, g[warmup:], gm[warmup:])
+        d.synchronize()
+        el = time.perf_counter() - t0
+        d.set_timing(True)
+        d.rt_steps(min(steps, 200), g[:min(steps, 200)], gm[:min(steps, 200)])
+        tm = d.timing()
+        d.set_timing(False)
+        mb = model_bytes(nx, ny, nz, fp64=prec == hip.GPE_FP64)
+        out[name] = {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "plane_pass_us": tm["plane_us"],
+                     "x_pass_us": tm["line_us"],
+                     "step_roofline": {"model_bytes_per_step": mb, "achieved_GBps": mb * steps / el / 1e9,
+                                       "frac": mb * steps / el / HBM_PEAK}}
+        d.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,11 +193,16 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="end with one device-to-device copy of this many MiB (the known-size "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE counter runs)")
+    ap.add_argument("--large", action="store_true",
+                    help="time the 256x256x64 ramped real-time workload (pencil plane pass) instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one solver instance on one GPU; --gpus must be 1 "
               "(multi-GPU slab decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.large:
+        print(json.dumps(large_rt(args.steps, args.warmup, args.fp64)))
+        return
 
     from gpesolver_amd import hip
 

@@ -190,6 +190,10 @@ template <typename T, int SIGN, bool EX>
 struct Dft<T, 16, SIGN, EX> {
     __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 4, 4, SIGN, EX>(v); }
 };
+template <typename T, int SIGN, bool EX>
+struct Dft<T, 32, SIGN, EX> {
+    __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 4, 8, SIGN, EX>(v); }
+};
 
 constexpr int pick_radix(int L) {
     return (L % 16 == 0) ? 16

@@ -1519,9 +1519,9 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
 // so a forward / multiply / inverse line costs two LDS exchanges.  Global
 // loads and stores go straight to / from registers; lines are the fastest
 // lane index, so neighbouring lanes touch neighbouring kz / k (contiguous).
-template <typename T, int L, int LPB, bool POISSON>
+template <typename T, int L, int LPB, bool POISSON, int A = 16>
 __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>* E, cplx<T>* TB) {
-    constexpr int A = 16, B = L / A, R = A / B;
+    constexpr int B = L / A, R = A / B;
     constexpr int LS = LPB + 1;  // padded row stride of the exchange area
     static_assert(L % A == 0 && A % B == 0, "line split");
     const int nx = a.nx, nz = a.nz;
@@ -1633,14 +1633,19 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
     }
 }
 
+// Register x pass shapes: A values per thread, B = L / A threads per line
+// (A % B == 0).  The 512-point Poisson lines of nx = 256 take A = 32.
 template <int NX>
 struct XReg {
-    static constexpr int LP = 2 * NX, LK = NX;          // Poisson / kinetic line lengths
-    static constexpr int LPB_P = 256 / (LP / 16);       // lines per 256-thread block
-    static constexpr int LPB_K = 256 / (LK / 16);
+    static constexpr int LP = 2 * NX, LK = NX;            // Poisson / kinetic line lengths
+    static constexpr int AP = LP / 16 > 16 ? 32 : 16, AK = 16;
+    static constexpr int LPB_P = 256 / (LP / AP);         // lines per 256-thread block
+    static constexpr int LPB_K = 256 / (LK / AK);
+    // exchange area: L (LPB + 1) complex
+    static constexpr int EXA = LP * (LPB_P + 1) > LK * (LPB_K + 1) ? LP * (LPB_P + 1) : LK * (LPB_K + 1);
 };
 
-// LDS: exchange area (16 * 256 + 256 complex) + tables (Poisson 2 * 2NX,
+// LDS: exchange area (XReg::EXA complex) + tables (Poisson 2 * 2NX,
 // kinetic 3 * NX complex).
 // The kinetic tiles (twice the loads of a zero-padded Poisson tile) take the
 // first block indices so that they are dispatched first.
@@ -1648,12 +1653,12 @@ template <typename T, int NX>
 __global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_k) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
-    cplx<T>* TB = E + 16 * 256 + 256;  // exchange area: L * (LPB + 1) <= 4096 + 256 elements
     using X = XReg<NX>;
+    cplx<T>* TB = E + X::EXA;
     if ((int)blockIdx.x < nblk_k)
-        xline_reg<T, X::LK, X::LPB_K, false>(a, blockIdx.x, E, TB);
+        xline_reg<T, X::LK, X::LPB_K, false, X::AK>(a, blockIdx.x, E, TB);
     else
-        xline_reg<T, X::LP, X::LPB_P, true>(a, blockIdx.x - nblk_k, E, TB);
+        xline_reg<T, X::LP, X::LPB_P, true, X::AP>(a, blockIdx.x - nblk_k, E, TB);
 }
 
 // Batched launches, one per grid shape: one block per work item (instance,
@@ -1686,11 +1691,11 @@ __global__ void __launch_bounds__(256) k_xpass_reg_group(const LineArgs<T>* __re
                                                          const int2* __restrict__ work) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
-    cplx<T>* TB = E + 16 * 256 + 256;
     using X = XReg<NX>;
+    cplx<T>* TB = E + X::EXA;
     const int2 w = work[blockIdx.x];
     const LineArgs<T> a = args[w.x];
-    xline_reg<T, X::LP, X::LPB_P, true>(a, w.y, E, TB);
+    xline_reg<T, X::LP, X::LPB_P, true, X::AP>(a, w.y, E, TB);
 }
 
 // ---------------------------------------------------------------------------
@@ -2291,7 +2296,7 @@ static int launch_line_reg(gpe_handle* h, const LineArgs<T>& a, bool poisson, bo
     const int np = poisson ? (2 * h->p.ny * h->p.nz + X::LPB_P - 1) / X::LPB_P : 0;
     const int nk = kinetic ? (h->p.ny * h->p.nz + X::LPB_K - 1) / X::LPB_K : 0;
     if (np + nk == 0) return 0;
-    const size_t lds = (size_t)(16 * 256 + 256 + 4 * NX) * sizeof(cplx<T>);
+    const size_t lds = (size_t)(X::EXA + 4 * NX) * sizeof(cplx<T>);
     hipEvent_t e0, e1;
     kernel_events(h, 1, &e0, &e1);
     hipExtLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, e0, e1, 0, a, nk);
@@ -2310,6 +2315,7 @@ static int launch_line(gpe_handle* h, bool poisson, bool kinetic, void* s1 = nul
     if (!h->force_generic && h->line_reg) {
         if (nx == 128) return launch_line_reg<T, 128>(h, a, poisson, kinetic);
         if (nx == 64) return launch_line_reg<T, 64>(h, a, poisson, kinetic);
+        if (nx == 256) return launch_line_reg<T, 256>(h, a, poisson, kinetic);
     }
     if (!h->force_generic) {
 #define GPE_LN(NX) \
@@ -2796,16 +2802,17 @@ static int batch_setup(gpe_batch* b) {
         for (auto& g : b->lgroups) seen = seen || g.a == h->p.nx;
         if (seen) continue;
         const int nx = h->p.nx;
-        const bool reg = !gen && h->line_reg && (nx == 128 || nx == 64);
+        const bool reg = !gen && h->line_reg && (nx == 128 || nx == 64 || nx == 256);
         gpe_batch::Group g{nx, 0, reg ? 1 : 0, (int)lw.size(), 0, 0};
         for (size_t r = q; r < na; ++r) {
             gpe_handle* hr = b->inst[b->active[r]];
             if (hr->p.nx != nx) continue;
             int nt = la[r].ntile_p;
             if (reg) {
-                const int lpb = nx == 128 ? XReg<128>::LPB_P : XReg<64>::LPB_P;
+                const int lpb = nx == 128 ? XReg<128>::LPB_P : nx == 64 ? XReg<64>::LPB_P : XReg<256>::LPB_P;
+                const int exa = nx == 128 ? XReg<128>::EXA : nx == 64 ? XReg<64>::EXA : XReg<256>::EXA;
                 nt = (2 * hr->p.ny * hr->p.nz + lpb - 1) / lpb;
-                g.lds = (size_t)(16 * 256 + 256 + 4 * nx) * sizeof(cplx<T>);
+                g.lds = (size_t)(exa + 4 * nx) * sizeof(cplx<T>);
             } else {
                 g.lds = std::max(g.lds, hr->line_lds);
             }
@@ -2873,6 +2880,9 @@ static int batch_line(gpe_batch* b) {
             done = true;
         } else if (g.kind == 1 && g.a == 64) {
             hipLaunchKernelGGL((k_xpass_reg_group<T, 64>), dim3(g.cnt), dim3(256), g.lds, b->stream, args, w);
+            done = true;
+        } else if (g.kind == 1 && g.a == 256) {
+            hipLaunchKernelGGL((k_xpass_reg_group<T, 256>), dim3(g.cnt), dim3(256), g.lds, b->stream, args, w);
             done = true;
         }
         if (!done && !gen) {
@@ -3335,7 +3345,7 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
         if (gpe_create(&h->p, h->prec, &s)) return fail(h, std::string("autotune scratch instance: ") + g_create_err);
         std::vector<double> psi(2 * h->N, 1.0);
         int rc = gpe_load_buffer(s, psi.data());
-        const bool reg_ok = !s->force_generic && (h->p.nx == 128 || h->p.nx == 64);
+        const bool reg_ok = !s->force_generic && (h->p.nx == 128 || h->p.nx == 64 || h->p.nx == 256);
         // graph replay of the steady state, or direct launches (GPE_NO_GRAPH=1
         // leaves only the latter): which one is faster depends on the grid
         const bool graphs_ok = s->use_graphs;
