@@ -199,10 +199,23 @@ __device__ double block_sum(double v, double* red) {
 // Sum of the per-plane partials, same order in every block.
 // Lane l of the first wave adds p[l], p[l+64], ... and a fixed shuffle tree
 // combines the lanes.
-__device__ double sum_partials(const double* p, int n, double* red) {
+// The first wave's loads of the first 64 kPartQ partials, issued together
+// (reduce_partials adds them, then any further ones, in index order).
+constexpr int kPartQ = 4;
+__device__ __forceinline__ void load_partials(const double* p, int n, double (&v)[kPartQ]) {
+#pragma unroll
+    for (int u = 0; u < kPartQ; ++u) {
+        v[u] = 0.0;
+        if (threadIdx.x < 64 && (int)threadIdx.x + 64 * u < n) v[u] = p[threadIdx.x + 64 * u];
+    }
+}
+__device__ double reduce_partials(const double* p, int n, const double (&v)[kPartQ], double* red) {
     if (threadIdx.x < 64) {
         double s = 0.0;
-        for (int t = threadIdx.x; t < n; t += 64) s += p[t];
+#pragma unroll
+        for (int u = 0; u < kPartQ; ++u)
+            if ((int)threadIdx.x + 64 * u < n) s += v[u];
+        for (int t = threadIdx.x + 64 * kPartQ; t < n; t += 64) s += p[t];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
         if (threadIdx.x == 0) red[0] = s;
@@ -211,6 +224,11 @@ __device__ double sum_partials(const double* p, int n, double* red) {
     double r = red[0];
     __syncthreads();
     return r;
+}
+__device__ double sum_partials(const double* p, int n, double* red) {
+    double v[kPartQ];
+    load_partials(p, n, v);
+    return reduce_partials(p, n, v, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,6 +645,11 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
     cplx<T> sy[kRegY2 ? 16 : 1];  // S1 column of the register y pass: S1[B a + b][i][c]
     cplx<T> pre[PRE];
     T vpre[PRE];
+    // tables staged to LDS: every load of this thread is issued before its
+    // first LDS store, so they cost one memory round trip (a load / store loop
+    // per table would wait for each table in turn)
+    constexpr int TQ = kStage ? 2 : 1;  // twiddles per thread in that round trip
+    cplx<T> tq[TQ], kq[4];
     if constexpr (kStage) {
         if (mode & M_PINV) {
             if constexpr (kRegY2) {
@@ -661,18 +684,67 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 if (need_v) vpre[q] = a.V[plane_off + p];
             }
         }
+        static_assert(Pol::kNY <= NT && Pol::kNZ <= NT, "one kinetic table element per thread");
+        const int t = threadIdx.x;
+        const bool kin = (mode & M_KIN) != 0;
+#pragma unroll
+        for (int q = 0; q < TQ; ++q) {
+            tq[q] = gpe::mk(T(0), T(0));
+            if (t + q * NT < a.tw_n) tq[q] = a.tw[t + q * NT];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) kq[q] = gpe::mk(T(0), T(0));
+        if (kin) {
+            if (t < nz) {
+                kq[0] = a.ez[t];
+                kq[3] = a.twlz[t];
+            }
+            if (t < ny) {
+                kq[1] = a.ey[t];
+                kq[2] = a.twly[t];
+            }
+        }
+    }
+
+    // partial norms of the previous pass, loaded with the staging loads
+    double pn[kPartQ];
+    if (mode & M_NORM_IN) load_partials(a.part_in, nx, pn);
+
+    // phase couplings of this step, issued while the staging loads are in
+    // flight (the table holds one step more than is read, so the four values
+    // load unconditionally; with a device step base it is a dependent pair)
+    double pg = a.pg, pgm = a.pgm;
+    if (a.coef && (mode & (M_PHASE_HALF | M_PHASE_FULL))) {
+        const int ts = (a.cbase ? *a.cbase : 0) + a.cidx;
+        const double* cp = a.coef + 2 * ts;
+        const double c0 = cp[0], c1 = cp[1], c2 = cp[2], c3 = cp[3];
+        const bool full = (mode & M_PHASE_FULL) != 0;
+        pg = full ? c0 + c2 : c0;
+        pgm = full ? c1 + c3 : c1;
+    }
+
+    if constexpr (kStage) {
+        const int t = threadIdx.x;
+        const bool kin = (mode & M_KIN) != 0;
         cplx<T>* TW = P + max(ny2 * nzh, ny * zs);
-        for (int m = threadIdx.x; m < a.tw_n; m += NT) TW[m] = a.tw[m];
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+            if (t + q * NT < a.tw_n) TW[t + q * NT] = tq[q];
+        for (int m = t + TQ * NT; m < a.tw_n; m += NT) TW[m] = a.tw[m];  // long x lines only
         tw = TW;
-        if (mode & M_KIN) {
+        if (kin) {
             cplx<T>* EZ = TW + a.tw_n;
             cplx<T>* EY = EZ + nz;
             cplx<T>* TWLY = EY + ny;
             cplx<T>* TWLZ = TWLY + ny;
-            for (int m = threadIdx.x; m < nz; m += NT) EZ[m] = a.ez[m];
-            for (int m = threadIdx.x; m < ny; m += NT) EY[m] = a.ey[m];
-            for (int m = threadIdx.x; m < ny; m += NT) TWLY[m] = a.twly[m];
-            for (int m = threadIdx.x; m < nz; m += NT) TWLZ[m] = a.twlz[m];
+            if (t < nz) {
+                EZ[t] = kq[0];
+                TWLZ[t] = kq[3];
+            }
+            if (t < ny) {
+                EY[t] = kq[1];
+                TWLY[t] = kq[2];
+            }
             ez = EZ;
             ey = EY;
             twly = TWLY;
@@ -680,24 +752,11 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         }
     }
 
-    // phase couplings of this step (a dependent pair of loads when they come
-    // from the device table: issued here, consumed after the Poisson finish)
-    double pg = a.pg, pgm = a.pgm;
-    if (a.coef && (mode & (M_PHASE_HALF | M_PHASE_FULL))) {
-        const int t = (a.cbase ? *a.cbase : 0) + a.cidx;
-        pg = a.coef[2 * t];
-        pgm = a.coef[2 * t + 1];
-        if (mode & M_PHASE_FULL) {
-            pg += a.coef[2 * t + 2];
-            pgm += a.coef[2 * t + 3];
-        }
-    }
-
     T phi_r[PPT];
     T rho_r[PPT];
 
     double s_in = 1.0;
-    if (mode & M_NORM_IN) s_in = 1.0 / sqrt(sum_partials(a.part_in, nx, red));
+    if (mode & M_NORM_IN) s_in = 1.0 / sqrt(reduce_partials(a.part_in, nx, pn, red));
 
     // ---- 1. Phi for this plane -------------------------------------------
     if ((mode & M_PINV) && kRegY2) {
@@ -2455,7 +2514,8 @@ static int ensure_coef(gpe_handle* h, size_t steps) {
     if (h->coef) GPE_CHECK(hipFree(h->coef));
     h->coef = nullptr;
     const size_t cap = std::max<size_t>(steps, 4096);
-    GPE_CHECK(hipMalloc(&h->coef, cap * 2 * sizeof(double)));
+    // one spare step: the plane pass loads a step's two coupling pairs at once
+    GPE_CHECK(hipMalloc(&h->coef, (cap + 1) * 2 * sizeof(double)));
     h->coef_cap = cap;
     return 0;
 }
