@@ -104,6 +104,7 @@ enum : int {
     M_IT = 512,        // imaginary-time stencil from `in` to `out`
     M_NORM_OUT = 1024, // per-plane sum |psi_out|^2 -> partials_out[i]
     M_NORM_IN = 2048,  // scale input by 1/sqrt(sum partials_in)
+    M_NO_PSI_OUT = 4096, // IT: the stencil result is not written (Poisson-start block of a split pass)
 };
 
 template <typename T>
@@ -929,7 +930,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 } else {
                     r = a.psii[plane_off + p];
                 }
-                a.out[plane_off + p] = r;
+                if (!(mode & M_NO_PSI_OUT)) a.out[plane_off + p] = r;
                 const T n2 = r.x * r.x + r.y * r.y;
                 rho_r[q] = n2;
                 nacc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
@@ -1541,9 +1542,10 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
 }
 
 // One block per x-plane, or (split) two: block i and block i + nx both
-// finish the Poisson solve and apply the phase of plane i; the first then
-// runs the kinetic z / y propagation (and any psi, fi3d or norm output),
-// the second starts the next Poisson solve.  The two halves of the plane
+// finish the Poisson solve and apply the phase (real time) or the stencil
+// (imaginary time) of plane i; the first then runs the kinetic z / y
+// propagation (and any psi, fi3d or norm output), the second starts the next
+// Poisson solve.  The two halves of the plane
 // work run on different CUs, at the price of reading the plane twice; the
 // outputs go to the other buffer of a ping-pong pair (s1o, tko) so no
 // block overwrites an input another block may not have read yet.
@@ -1556,7 +1558,7 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
         if (i < a.nx)
             mode &= ~M_RHO_FWD;
         else
-            mode &= ~(M_KIN | M_OUT | M_PHI_STORE | M_NORM_OUT);
+            mode = (mode & ~(M_KIN | M_OUT | M_PHI_STORE | M_NORM_OUT)) | M_NO_PSI_OUT;
         i = i < a.nx ? i : i - a.nx;
     }
     plane_body<T, NT, Pol>(a, i, mode, lds_raw, red);
@@ -1936,7 +1938,7 @@ struct gpe_handle {
     void* phi = nullptr;
     void* s1 = nullptr;
     void* tk = nullptr;
-    void* s1b = nullptr;  // second S1 / Tk of the split real-time plane pass (allocated on first use)
+    void* s1b = nullptr;  // second S1 / Tk of the split plane pass (allocated on first use)
     void* tkb = nullptr;
     void* tw = nullptr;
     void* twl = nullptr;  // lo parts of the W_nx, W_ny, W_nz twiddles, [nx][ny][nz]
@@ -1952,7 +1954,7 @@ struct gpe_handle {
     bool norm_pending = false;  // true: normalised psi = store[cur]/sqrt(sum part[pcur])
     bool force_generic = false; // GPE_FORCE_GENERIC=1: run-time-size kernels only
     bool line_reg = true;       // register-resident x pass (GPE_LINE_LDS=1 disables)
-    bool plane_split = true;    // two blocks per plane in steady real time (GPE_NO_SPLIT=1 disables)
+    bool plane_split = true;    // two blocks per plane in steady RT / IT steps (GPE_NO_SPLIT=1 disables)
     bool pencil = false;        // plane beyond one block's LDS: the pencil plane pass
     void* pen_gw = nullptr;     // its per-plane intermediates (N complex each)
     void* pen_h = nullptr;
@@ -2538,11 +2540,26 @@ static int capture(gpe_handle* h, hipGraphExec_t* out, F&& body) {
     return 0;
 }
 
-// One steady-state IT step: Poisson x pass, then the plane pass (finish
-// Poisson, stencil, norm partials, start next Poisson).  Flips cur / pcur.
+// S1 / Tk buffers of step parity par: the split plane pass reads pair par
+// and writes pair 1 - par (the unsplit pass works in place on s1 / tk).
+// Every loop starts on parity 0 (its prologue writes s1).
+static void* rt_s1(gpe_handle* h, int par) { return (h->plane_split && par) ? h->s1b : h->s1; }
+static void* rt_tk(gpe_handle* h, int par) { return (h->plane_split && par) ? h->tkb : h->tk; }
+
+static int ensure_split_buffers(gpe_handle* h, bool tk) {
+    if (!h->plane_split) return 0;
+    const size_t cs = csize(h);
+    if (!h->s1b) GPE_CHECK(hipMalloc(&h->s1b, (size_t)2 * h->p.ny * h->p.nx * h->p.nz * cs));
+    if (tk && !h->tkb) GPE_CHECK(hipMalloc(&h->tkb, h->N * cs));
+    return 0;
+}
+
+// One IT step: Poisson x pass, then the plane pass (finish Poisson, stencil,
+// norm partials, start next Poisson; split into two blocks per plane except
+// on the last step, which starts no Poisson solve).  Flips cur / pcur and,
+// split, the S1 parity.
 template <typename T>
-static int it_step_launch(gpe_handle* h, bool last) {
-    if (launch_line<T>(h, true, false)) return 1;
+static PlaneArgs<T> it_plane_args(gpe_handle* h, bool last, int par) {
     int mode = M_PINV | M_NORM_IN | M_IT | M_NORM_OUT | M_RHO_FWD;
     if (last) mode = (mode & ~M_RHO_FWD) | M_PHI_STORE;
     PlaneArgs<T> a = plane_args<T>(h, mode);
@@ -2550,15 +2567,29 @@ static int it_step_launch(gpe_handle* h, bool last) {
     a.out = (cplx<T>*)h->store[1 - h->cur];
     a.part_in = h->part[h->pcur];
     a.part_out = h->part[1 - h->pcur];
-    if (launch_plane<T>(h, a)) return 1;
+    a.s1 = a.s1o = (cplx<T>*)rt_s1(h, par);
+    if (h->plane_split && !last) {
+        a.split = 1;
+        a.s1o = (cplx<T>*)rt_s1(h, 1 - par);
+    }
+    return a;
+}
+
+template <typename T>
+static int it_step_launch(gpe_handle* h, bool last, int& par) {
+    if (launch_line<T>(h, true, false, rt_s1(h, par))) return 1;
+    if (launch_plane<T>(h, it_plane_args<T>(h, last, par))) return 1;
     h->cur = 1 - h->cur;
     h->pcur = 1 - h->pcur;
+    par = 1 - par;
     return 0;
 }
 
 template <typename T>
 static int do_it_iters(gpe_handle* h, long n) {
     if (n <= 0) return 0;
+    if (ensure_split_buffers(h, false)) return 1;
+    int par = 0;
     // prologue: partials of the current state + Poisson start of |store|^2
     PlaneArgs<T> a = plane_args<T>(h, M_RHO_FWD | M_NORM_OUT);
     a.in = (const cplx<T>*)h->store[h->cur];
@@ -2576,8 +2607,9 @@ static int do_it_iters(gpe_handle* h, long n) {
         if (!h->it_graph[gi]) {
             const int c0 = h->cur, p0 = h->pcur;
             const int rc = capture(h, &h->it_graph[gi], [&]() -> int {
+                int cpar = par;  // kGraphSteps is even: a replay ends on the parity it started on
                 for (int t = 0; t < kGraphSteps; ++t)
-                    if (it_step_launch<T>(h, false)) return 1;
+                    if (it_step_launch<T>(h, false, cpar)) return 1;
                 return 0;
             });
             h->cur = c0;
@@ -2594,29 +2626,19 @@ static int do_it_iters(gpe_handle* h, long n) {
     h->kn[0] = h->kn[1] = 0;
     h->tim_on = true;  // time the steady-state launches
     for (long t = 0; t < steady; ++t) {
-        if (launch_line<T>(h, true, false)) return 1;
+        if (launch_line<T>(h, true, false, rt_s1(h, par))) return 1;
         ev_record(h, ne++);
-        int mode = M_PINV | M_NORM_IN | M_IT | M_NORM_OUT | M_RHO_FWD;
-        a = plane_args<T>(h, mode);
-        a.in = (const cplx<T>*)h->store[h->cur];
-        a.out = (cplx<T>*)h->store[1 - h->cur];
-        a.part_in = h->part[h->pcur];
-        a.part_out = h->part[1 - h->pcur];
-        if (launch_plane<T>(h, a)) return 1;
+        if (launch_plane<T>(h, it_plane_args<T>(h, false, par))) return 1;
         ev_record(h, ne++);
         h->cur = 1 - h->cur;
         h->pcur = 1 - h->pcur;
+        par = 1 - par;
     }
     h->tim_on = false;
-    if (it_step_launch<T>(h, true)) return 1;
+    if (it_step_launch<T>(h, true, par)) return 1;
     h->norm_pending = true;
     return finish_timing(h, ne);
 }
-
-// S1 / Tk buffers of real-time step parity par: the split plane pass reads
-// pair par and writes pair 1 - par (the unsplit pass works in place on s1 / tk).
-static void* rt_s1(gpe_handle* h, int par) { return (h->plane_split && par) ? h->s1b : h->s1; }
-static void* rt_tk(gpe_handle* h, int par) { return (h->plane_split && par) ? h->tkb : h->tk; }
 
 // Steady-state RT plane pass for step t -> t+1 (coupling from the table).
 template <typename T>
@@ -2650,11 +2672,7 @@ static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm)
         cf[2 * t + 1] = (gm ? gm[t] : h->gamma) * p.w_15 / pow(dV, 1.5);
     }
     GPE_CHECK(hipMemcpyAsync(h->coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    if (h->plane_split && !h->s1b) {
-        const size_t cs = 2 * sizeof(T);
-        GPE_CHECK(hipMalloc(&h->s1b, (size_t)2 * p.ny * p.nx * p.nz * cs));
-        GPE_CHECK(hipMalloc(&h->tkb, h->N * cs));
-    }
+    if (ensure_split_buffers(h, true)) return 1;
     // prologue: first half phase with the current fi3d, Poisson start, z/y kinetic
     PlaneArgs<T> a = plane_args<T>(h, M_PHI_LOAD | M_PHASE_HALF | M_RHO_FWD | M_KIN);
     a.in = (const cplx<T>*)h->store[h->cur];

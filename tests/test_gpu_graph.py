@@ -68,16 +68,22 @@ def test_batch_graph_equals_direct(monkeypatch):
 @pytest.mark.parametrize("prec", [hip.GPE_FP64, hip.GPE_FP32])
 @pytest.mark.parametrize("shape", [(32, 24, 12), (64, 64, 16)])
 def test_split_plane_pass_equals_single(monkeypatch, prec, shape):
-    """Two blocks per plane with ping-pong S1 / Tk (the default real-time
-    plane pass) is bitwise the one-block-per-plane pass (GPE_NO_SPLIT=1),
-    for step counts of both parities, with and without graph replay."""
+    """Two blocks per plane with ping-pong S1 / Tk (the default real-time and
+    imaginary-time plane pass) is bitwise the one-block-per-plane pass
+    (GPE_NO_SPLIT=1), for step counts of both parities, with and without
+    graph replay, and through the energy-checked IT loop."""
     p, V, psi = case(shape)
     outs = {}
     for split in ("0", "1"):
         monkeypatch.setenv("GPE_NO_SPLIT", split)
         d = dev_for(p, V, psi, prec)
-        d.calc_fi3d()
         r = []
+        for n in (1, 2, 67, 130):
+            d.it_steps(n)
+            r.append(d.export_psi())
+        steps, epa = d.it_until(450, 100, 1e-12)
+        r += [d.export_psi(), np.array([steps, epa])]
+        d.calc_fi3d()
         for n in (1, 2, 67, 130):
             d.rt_steps(n)
             r.append(d.export_psi())
