@@ -158,3 +158,45 @@ def compare(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID
         "plain_e_per_atom": e_plain.tolist(),
         "warm_e_per_atom": e_warm.tolist(),
     }
+
+
+def compare_single(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID, precision=hip.GPE_FP32):
+    """Per-instance time to |dE/N| < tol on a single handle, from the
+    instance's parameters and potential: plain = building the MULTIPLE_GAUSS
+    state + imaginary time; warm = building the features, FNO inference (a
+    batch of one) + imaginary time from its prediction.  Each handle runs one
+    untimed 300-step pass first, so both timed runs replay the same captured
+    graphs."""
+    out = {"plain_s": [], "warm_s": [], "plain_steps": [], "warm_steps": [], "fallbacks": 0}
+    for d in insts:
+        p, V, psi0 = build(d, grid)
+        dev = hip.Device(p.to_gpe(), precision)
+        try:
+            dev.load_pote(V)
+            dev.load_buffer(psi0)
+            dev.it_until(300, 100, 0.0)
+            dev.synchronize()
+            t0 = time.perf_counter()
+            psi = P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2, sigma_cells=INIT_SIGMA_CELLS)
+            dev.load_buffer(psi)
+            steps, _ = dev.it_until(max_steps, 100, tol)
+            out["plain_s"].append(time.perf_counter() - t0)
+            out["plain_steps"].append(int(steps))
+            t0 = time.perf_counter()
+            y = model.forward(F.features(p, V, d["pot"])[None])
+            psi = F.warm_state(y[0], grid)
+            if not np.all(np.isfinite(psi)) or np.abs(psi).max() <= 0.0:
+                psi = P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2, sigma_cells=INIT_SIGMA_CELLS)
+                out["fallbacks"] += 1
+            dev.load_buffer(psi)
+            steps, _ = dev.it_until(max_steps, 100, tol)
+            out["warm_s"].append(time.perf_counter() - t0)
+            out["warm_steps"].append(int(steps))
+        finally:
+            dev.close()
+    tp, tw = np.array(out["plain_s"]), np.array(out["warm_s"])
+    r = tp / tw
+    out.update({"ratio_median": float(np.median(r)), "ratio_total": float(tp.sum() / tw.sum()),
+                "ratio_ge5_fraction": float(np.mean(r >= 5.0)), "plain_total_s": float(tp.sum()),
+                "warm_total_s": float(tw.sum())})
+    return out

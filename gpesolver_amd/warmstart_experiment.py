@@ -2,7 +2,9 @@
 layers, 16 modes per axis capped at nz/2+1 along z, BF16 GEMM operands with
 FP32 accumulation): generate ground states with the batched IT solver, train
 the network on the GPU, then time batched IT to |dE/N| < tol from the
-network's prediction against plain MULTIPLE_GAUSS, on held-out instances.
+network's prediction against plain MULTIPLE_GAUSS, on held-out instances,
+and the same per instance on single handles (time to convergence including
+that instance's inference).
 
     python -m gpesolver_amd.warmstart_experiment --n-train 2000 --n-test 64 --out result.json
 """
@@ -49,6 +51,7 @@ def main(argv=None):
     hist = W.train(model, X, T, args.epochs, args.lr, args.batch, log=lambda m: print(m, flush=True))
     t_train = time.perf_counter() - t0
     res = W.compare(model, test_i, args.tol, args.max_steps, grid=grid)
+    res["single"] = W.compare_single(model, test_i, args.tol, args.max_steps, grid=grid)
     res.update({"grid": list(grid), "n_train": args.n_train, "epochs": args.epochs, "train_mse": hist,
                 "generate_s": t_gen, "train_s": t_train, "train_steps_median": float(np.median(S)),
                 "fno": {"width": args.width, "layers": args.layers, "modes": args.modes,
