@@ -190,6 +190,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64", action="store_true", help="also time the FP64 path")
     ap.add_argument("--it-max-steps", type=int, default=20000, help="cap of the IT time-to-convergence leg")
+    ap.add_argument("--rt-only", action="store_true",
+                    help="profiling runs: the RT leg only, on the default launch shapes (split plane pass, "
+                         "register x pass: what the autotuner picks at this grid), no autotune and no IT legs, "
+                         "so every steady-state launch in a trace or counter run is an RT one")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="end with one device-to-device copy of this many MiB (the known-size "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -208,7 +212,8 @@ def main():
 
     nx, ny, nz = GRID
     p, dev, V, psi = build_case(hip.GPE_FP32)
-    tuned = dev.autotune(200)  # launch-shape variant for this grid (scratch instance)
+    # launch-shape variant for this grid (scratch instance)
+    tuned = None if args.rt_only else dev.autotune(200)
     # warm-up
     dev.rt_steps(args.warmup)
     dev.synchronize()
@@ -228,6 +233,15 @@ def main():
     kern = {"plane": tm["plane_us"], "line": tm["line_us"]}
     dom = max(kern, key=kern.get)
     achieved = kb[dom] / (kern[dom] * 1e-6)
+
+    if args.rt_only:
+        dev.close()
+        if args.calib_copy_mib > 0:
+            calib_copy(args.calib_copy_mib << 20)
+        print(json.dumps({"metric": "rt_steps_per_s", "value": steps_per_s, "unit": "steps/s", "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": 1e3 / steps_per_s, "rt_only": True,
+                          "avg_launch_us": kern, "calib_copy_mib": args.calib_copy_mib}), flush=True)
+        return
 
     # IT throughput on the same grid (FP32), fused loop
     dev.it_steps(args.warmup)
