@@ -153,7 +153,7 @@ int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float
                   int N, int K, int conjA, int conjB);
 
 /* Autotuner (no reference counterpart).  Times the launch-shape variants of
- * the fused loops (one or two blocks per plane in the real-time plane pass;
+ * the fused loops (one or two blocks per plane in the steady-state plane pass;
  * register-resident or LDS x pass; graph-replayed or directly launched
  * steady state) over `steps` steps on a scratch instance of h's grid and
  * precision, and keeps the fastest for h (cached per grid and precision for
