@@ -1281,13 +1281,29 @@ __global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
         rho_r[q] = T(0);
         cv[q] = gpe::mk(T(0), T(0));
     }
-    // Phi of the tile rows: z C2R of G (Hermitian extension, kz = 0 zero)
+    // Phi of the tile rows: z C2R of G (Hermitian extension, kz = 0 zero).
+    // The first GQ loads of each thread are issued together (one round trip).
     if (mode & M_PINV) {
-        for (int e = threadIdx.x; e < rw * (nz + 1); e += NT) {
+        constexpr int GQ = 4;
+        const int ne = rw * (nz + 1);
+        cplx<T> gv[GQ];
+#pragma unroll
+        for (int q = 0; q < GQ; ++q) {
+            const int e = threadIdx.x + q * NT;
             const int r = e / (nz + 1), kz = e - r * (nz + 1);
-            const cplx<T> v = kz ? gw[(size_t)(y0 + r) * nz + kz - 1] : gpe::mk(T(0), T(0));
+            gv[q] = (e < ne && kz) ? gw[(size_t)(y0 + r) * nz + kz - 1] : gpe::mk(T(0), T(0));
+        }
+        auto put = [&](int e, cplx<T> v) {
+            const int r = e / (nz + 1), kz = e - r * (nz + 1);
             X[r * zs + kz] = v;
             if (kz >= 1 && kz < nz) X[r * zs + nz2 - kz] = gpe::mk(v.x, -v.y);
+        };
+#pragma unroll
+        for (int q = 0; q < GQ; ++q)
+            if (threadIdx.x + q * NT < ne) put(threadIdx.x + q * NT, gv[q]);
+        for (int e = threadIdx.x + GQ * NT; e < ne; e += NT) {
+            const int r = e / (nz + 1), kz = e - r * (nz + 1);
+            put(e, kz ? gw[(size_t)(y0 + r) * nz + kz - 1] : gpe::mk(T(0), T(0)));
         }
         __syncthreads();
         Pol::template z2<T, NT, +1>(X, rw, 1, zs, a);
@@ -1353,6 +1369,10 @@ __global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
             }
         }
     } else if (have_psi) {
+        // FP32: the phase argument in FP32 where it is small (as in the
+        // single-block plane pass), FP64 argument and reduction otherwise
+        const float tv = (float)(a.hdt * a.cv), tphi = (float)(a.hdt * a.cv * a.cdd), tg = (float)(a.hdt * pg),
+                    tgm = (float)(a.hdt * pgm);
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int p = threadIdx.x + q * NT;
@@ -1369,10 +1389,20 @@ __global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
                 T nrm = c.x * c.x + c.y * c.y;
                 if ((mode & (M_PHASE_HALF | M_PHASE_FULL)) && iint && j >= 1 && j <= ny - 2 && k >= 1 &&
                     k <= nz - 2) {
-                    const double dn = (double)nrm;
-                    const double veff =
-                        a.cv * ((double)a.V[e] + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
-                    c = gpe::phase_rot(c, -a.hdt * veff);
+                    bool done = false;
+                    if constexpr (sizeof(T) == 4) {
+                        const float t32 = tv * a.V[e] + tphi * phi_r[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
+                        if (fabsf(t32) <= 0.0625f) {
+                            c = gpe::phase_rot_small(c, -t32);
+                            done = true;
+                        }
+                    }
+                    if (!done) {
+                        const double dn = (double)nrm;
+                        const double veff =
+                            a.cv * ((double)a.V[e] + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
+                        c = gpe::phase_rot(c, -a.hdt * veff);
+                    }
                     nrm = c.x * c.x + c.y * c.y;
                 }
                 cv[q] = c;
