@@ -1208,7 +1208,7 @@ struct PencilPolCT {
     }
 };
 // Pencil shapes with compile-time kernels: (ny, nz, columns, rows per tile).
-#define GPE_PENCIL_SHAPES(X) X(256, 64, 8, 16)
+#define GPE_PENCIL_SHAPES(X) X(256, 64, 8, 32)
 
 // phase couplings of this step (as in plane_body)
 template <typename T>
