@@ -39,6 +39,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gpehip.h"
@@ -1752,6 +1753,165 @@ __global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_k) {
         xline_reg<T, X::LP, X::LPB_P, true, X::AP>(a, blockIdx.x - nblk_k, E, TB);
 }
 
+// Register y passes of the pencil plane pass (FP32, compiled ny): the
+// four-step of the register x pass on y lines (kz columns of one plane, LPB
+// columns per 256-thread block), one direction at a time: YR_FWD forward
+// (inputs from index nin on are zero), YR_INV inverse (outputs from index
+// nout on are not stored), YR_KIN forward, kinetic phase, inverse (psi
+// transforms: compensated twiddles and constants).  in / out point at the
+// thread's line; istr / ostr are the element strides.  tw, twl, ek are LDS
+// tables of the line length (twl and ek: YR_KIN only).
+enum { YR_FWD = 0, YR_INV = 1, YR_KIN = 2 };
+template <typename T, int L, int LPB, int DIR, int A>
+__device__ __forceinline__ void yline_reg(const cplx<T>* in, size_t istr, int nin, cplx<T>* out, size_t ostr, int nout,
+                                          bool valid, cplx<T>* E, const cplx<T>* tw, const cplx<T>* twl,
+                                          const cplx<T>* ek) {
+    constexpr int B = L / A, R = A / B, LS = LPB + 1;
+    constexpr bool EX = DIR == YR_KIN;
+    static_assert(L % A == 0 && A % B == 0 && LPB * B == 256, "y line split");
+    const int ll = threadIdx.x % LPB, b = threadIdx.x / LPB;
+    cplx<T> w[R][B];
+    if constexpr (DIR != YR_INV) {
+        cplx<T> v[A];
+#pragma unroll
+        for (int q = 0; q < A; ++q) {
+            v[q] = gpe::mk(T(0), T(0));
+            if (valid && B * q + b < nin) v[q] = in[(size_t)(B * q + b) * istr];
+        }
+        gpe::ct::Dft<T, A, -1, EX>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < A; ++k1) {
+            if constexpr (EX)
+                v[k1] = tw_psi<T, -1>(v[k1], tw, twl, (b * k1) % L);
+            else
+                v[k1] = gpe::cmul(v[k1], tw[(b * k1) % L]);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = v[k1];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k1 = b + B * r;
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * LS + ll];
+            gpe::ct::Dft<T, B, -1, EX>::run(w[r]);
+        }
+        if constexpr (DIR == YR_FWD) {
+            if (valid) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int k2 = 0; k2 < B; ++k2) out[(size_t)(b + B * r + A * k2) * ostr] = w[r][k2];
+            }
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::kin_mul(w[r][k2], ek[b + B * r + A * k2], 1.0 / L);
+        __syncthreads();  // every read of E is done before it is rewritten
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2)
+                w[r][k2] = valid ? in[(size_t)(b + B * r + A * k2) * istr] : gpe::mk(T(0), T(0));
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k1 = b + B * r;
+        gpe::ct::Dft<T, B, +1, EX>::run(w[r]);
+#pragma unroll
+        for (int bb = 1; bb < B; ++bb) {
+            if constexpr (EX)
+                w[r][bb] = tw_psi<T, +1>(w[r][bb], tw, twl, (bb * k1) % L);
+            else
+                w[r][bb] = gpe::cmulc(w[r][bb], tw[(bb * k1) % L]);
+        }
+#pragma unroll
+        for (int bb = 0; bb < B; ++bb) E[(k1 * B + bb) * LS + ll] = w[r][bb];
+    }
+    __syncthreads();
+    cplx<T> v[A];
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * LS + ll];
+    gpe::ct::Dft<T, A, +1, EX>::run(v);
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < A; ++q)
+            if (B * q + b < nout) out[(size_t)(B * q + b) * ostr] = v[q];
+    }
+}
+
+// Shapes of the register pencil y passes: 2NY-point lines as 32 x 16, NY-point
+// lines as 16 x 16, 16 kz columns per block either way.
+template <int NY>
+struct PenY {
+    static constexpr int L2 = 2 * NY, L1 = NY, A2 = 32, A1 = 16, LPB = 16;
+    static constexpr int EXA = L2 * (LPB + 1);  // exchange area (complex)
+    static constexpr int TAB = L2 > 3 * L1 ? L2 : 3 * L1;  // tables (complex)
+};
+
+// inverse y2 of S1 columns -> G (the k_pencil_ci step)
+template <typename T, int NY>
+__global__ void __launch_bounds__(256) k_pencil_yci(PlaneArgs<T> a) {
+    using Y = PenY<NY>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
+    cplx<T>* TB = E + Y::EXA;
+    const int nx = a.nx, nz = a.nz, ncb = (nz + Y::LPB - 1) / Y::LPB;
+    const int i = blockIdx.x / ncb, c = (blockIdx.x - i * ncb) * Y::LPB + (int)threadIdx.x % Y::LPB;
+    const bool valid = c < nz;
+    for (int m = threadIdx.x; m < Y::L2; m += 256) TB[m] = a.tw[a.py2.tw + m];
+    __syncthreads();
+    const int cc = valid ? c : 0;
+    yline_reg<T, Y::L2, Y::LPB, YR_INV, Y::A2>(a.s1 + (size_t)i * nz + cc, (size_t)nx * nz, Y::L2,
+                                              a.pen_gw + (size_t)i * NY * nz + cc, nz, NY, valid, E, TB, nullptr,
+                                              nullptr);
+}
+
+// blocks [0, nx ncb): y2 of H -> S1o (M_RHO_FWD); [nx ncb, 2 nx ncb): y
+// kinetic of W -> Tk (M_KIN); the first block of each plane also sums the
+// plane's row-tile norm partials in order (M_NORM_OUT).  The k_pencil_cf step.
+template <typename T, int NY>
+__global__ void __launch_bounds__(256) k_pencil_ycf(PlaneArgs<T> a) {
+    using Y = PenY<NY>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
+    cplx<T>* TB = E + Y::EXA;
+    const int mode = a.mode;
+    const int nx = a.nx, nz = a.nz, ncb = (nz + Y::LPB - 1) / Y::LPB;
+    int blk = blockIdx.x;
+    const bool kin = blk >= nx * ncb;
+    if (kin) blk -= nx * ncb;
+    const int i = blk / ncb, c0 = (blk - i * ncb) * Y::LPB, c = c0 + (int)threadIdx.x % Y::LPB;
+    const bool valid = c < nz;
+    const int cc = valid ? c : 0;
+    const size_t plane_off = (size_t)i * NY * nz;
+    if (!kin && c0 == 0 && (mode & M_NORM_OUT) && threadIdx.x == 0) {
+        const int nrt = (NY + a.pen_r - 1) / a.pen_r;
+        double sum = 0.0;
+        for (int t = 0; t < nrt; ++t) sum += a.pen_part[(size_t)i * nrt + t];
+        a.part_out[i] = sum;
+    }
+    if (!kin && (mode & M_RHO_FWD)) {
+        for (int m = threadIdx.x; m < Y::L2; m += 256) TB[m] = a.tw[a.py2.tw + m];
+        __syncthreads();
+        yline_reg<T, Y::L2, Y::LPB, YR_FWD, Y::A2>(a.pen_h + plane_off + cc, nz, NY, a.s1o + (size_t)i * nz + cc,
+                                                  (size_t)nx * nz, Y::L2, valid, E, TB, nullptr, nullptr);
+    }
+    if (kin && (mode & M_KIN)) {
+        for (int m = threadIdx.x; m < Y::L1; m += 256) {
+            TB[m] = a.tw[a.py.tw + m];
+            TB[Y::L1 + m] = a.twly[m];
+            TB[2 * Y::L1 + m] = a.ey[m];
+        }
+        __syncthreads();
+        yline_reg<T, Y::L1, Y::LPB, YR_KIN, Y::A1>(a.pen_gw + plane_off + cc, nz, Y::L1, a.tko + (size_t)i * nz + cc,
+                                                  (size_t)nx * nz, Y::L1, valid, E, TB, TB + Y::L1, TB + 2 * Y::L1);
+    }
+}
+
 // Batched launches, one per grid shape: one block per work item (instance,
 // plane) / (instance, tile); each instance brings its own argument block.
 // A launch per shape keeps the single-instance kernels' register budget (the
@@ -1990,6 +2150,7 @@ struct gpe_handle {
     void* pen_h = nullptr;
     double* pen_part = nullptr;  // norm partial per (plane, row tile)
     int pen_c = 0, pen_r = 0;   // column / row tile shapes
+    bool pen_yreg = true;       // register y passes of the pencil pass (GPE_PENCIL_YREG=0: LDS column tiles)
     size_t pen_lds = 0;
     LinePlan px, py, pz, px2, py2, pz2;
     // launch shapes
@@ -2334,17 +2495,31 @@ static int launch_pencil(gpe_handle* h, const PlaneArgs<T>& a) {
     const int nct = (nz + h->pen_c - 1) / h->pen_c, nrt = (ny + h->pen_r - 1) / h->pen_r;
     const bool ci = (a.mode & M_PINV) != 0;
     const bool cf = (a.mode & (M_RHO_FWD | M_KIN | M_NORM_OUT)) != 0;
+    // register y passes: FP32, compiled pencil shapes with ny = 256
+    constexpr bool kYReg = sizeof(T) == 4 && !std::is_same<Pol, PencilPolGeneric>::value;
+    const bool yreg = kYReg && h->pen_yreg && ny == 256;
+    using Y = PenY<256>;
+    const int ncb = (nz + Y::LPB - 1) / Y::LPB;
+    const size_t ylds = (size_t)(Y::EXA + Y::TAB) * sizeof(cplx<T>);
     if (ci) {
-        hipExtLaunchKernelGGL((k_pencil_ci<T, kPencilThreads, Pol>), dim3(nx * nct), dim3(kPencilThreads), h->pen_lds,
-                              h->stream, e0, nullptr, 0, a);
+        if (yreg)
+            hipExtLaunchKernelGGL((k_pencil_yci<T, 256>), dim3(nx * ncb), dim3(256), ylds, h->stream, e0, nullptr, 0,
+                                  a);
+        else
+            hipExtLaunchKernelGGL((k_pencil_ci<T, kPencilThreads, Pol>), dim3(nx * nct), dim3(kPencilThreads),
+                                  h->pen_lds, h->stream, e0, nullptr, 0, a);
         GPE_CHECK(hipGetLastError());
     }
     hipExtLaunchKernelGGL((k_pencil_r<T, kPencilThreads, Pol>), dim3(nx * nrt), dim3(kPencilThreads), h->pen_lds,
                           h->stream, ci ? nullptr : e0, cf ? nullptr : e1, 0, a);
     GPE_CHECK(hipGetLastError());
     if (cf) {
-        hipExtLaunchKernelGGL((k_pencil_cf<T, kPencilThreads, Pol>), dim3(2 * nx * nct), dim3(kPencilThreads),
-                              h->pen_lds, h->stream, nullptr, e1, 0, a);
+        if (yreg)
+            hipExtLaunchKernelGGL((k_pencil_ycf<T, 256>), dim3(2 * nx * ncb), dim3(256), ylds, h->stream, nullptr, e1, 0,
+                                  a);
+        else
+            hipExtLaunchKernelGGL((k_pencil_cf<T, kPencilThreads, Pol>), dim3(2 * nx * nct), dim3(kPencilThreads),
+                                  h->pen_lds, h->stream, nullptr, e1, 0, a);
         GPE_CHECK(hipGetLastError());
     }
     return 0;
@@ -3201,6 +3376,8 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
         h->pen_c = std::min(p->nz, pmaxe / (2 * p->ny) - 1);
         h->pen_r = std::min(std::min(p->ny, kPencilPPT * kPencilThreads / p->nz), pmaxe / (2 * p->nz + 1));
         // GPE_PENCIL_TILES="c,r" caps the tile shapes (tests: partial last tiles)
+        const char* yr = getenv("GPE_PENCIL_YREG");
+        h->pen_yreg = !(yr && yr[0] == '0');
         const char* pt = getenv("GPE_PENCIL_TILES");
         int tc = 0, tr = 0;
         if (pt && sscanf(pt, "%d,%d", &tc, &tr) == 2 && tc > 0 && tr > 0) {
