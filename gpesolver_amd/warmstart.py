@@ -10,7 +10,7 @@ Pipeline:
      GPU forward / backward kernels and Adam;
   4. warm start: psi0 = FNO prediction, then plain imaginary time refines it
      to the same criterion.  Plain MULTIPLE_GAUSS imaginary time stays the
-     fallback (non-finite or empty prediction) and the judge (the warm run
+     fallback (non-finite or empty prediction) and the acceptance check (the warm run
      must reach the plain run's E/N).
 """
 from __future__ import annotations
@@ -138,7 +138,7 @@ def compare(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID
     warm_init, fb = predict_states(model, insts, grid)
     t_inf = time.perf_counter() - t0
     s_warm, e_warm, _, t_warm = run_batch(insts, warm_init, tol, max_steps, grid=grid)
-    # judge: the warm run must reach the plain run's energy
+    # acceptance check: the warm run must reach the plain run's energy
     de = np.abs(e_warm - e_plain) / np.abs(e_plain)
     return {
         "instances": len(insts),
