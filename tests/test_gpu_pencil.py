@@ -141,6 +141,40 @@ def test_large_rt_first_steps_against_oracle(oracle, monkeypatch):
         dev.close()
 
 
+def test_large_it_and_poisson_against_oracle(oracle, monkeypatch):
+    """256x256x64: Poisson hook, 3 fused imaginary-time steps and the energies against the oracle,
+    FP64, FP32 with the register y passes (the default at ny = 256) and FP32 with the LDS column
+    tiles; the two FP32 column paths also against each other."""
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    p, op, V, psi = large_case(oracle)
+    orc = oracle.Oracle(op)
+    orc.load_pote(V)
+    orc.load_buffer(psi)
+    orc.calc_fi3d()
+    fi0 = orc.fi3d.copy()
+    orc.it_steps(3)
+    eo = orc.energy()
+    out = {}
+    for prec, yreg in ((hip.GPE_FP64, "1"), (hip.GPE_FP32, "1"), (hip.GPE_FP32, "0")):
+        monkeypatch.setenv("GPE_PENCIL_YREG", yreg)
+        dev = dev_of(p, prec, V, psi)
+        dev.calc_fi3d()
+        rf = rel(dev.export_fi3d(), fi0)
+        dev.it_steps(3)
+        out[(prec, yreg)] = dev.export_psi()
+        r = rel(out[(prec, yreg)], orc.psi)
+        e = dev.energy()
+        de = abs(e["e_total"] - eo["e_total"]) / abs(eo["e_total"])
+        print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} (GPE_PENCIL_YREG={yreg}) vs oracle: "
+              f"fi3d {rf:.3e}, psi after 3 IT steps {r:.3e}, E {de:.3e}", flush=True)
+        fp64 = prec == hip.GPE_FP64
+        assert rf < (1e-11 if fp64 else 1e-5)
+        assert r < (1e-10 if fp64 else 2e-6)
+        assert de < (1e-10 if fp64 else 1e-6)
+        dev.close()
+    assert rel(out[(hip.GPE_FP32, "1")], out[(hip.GPE_FP32, "0")]) < 1e-6
+
+
 def test_large_rt_500_steps_fp32_vs_fp64():
     p, _, V, psi = large_case()
     devs = {prec: dev_of(p, prec, V, psi) for prec in (hip.GPE_FP64, hip.GPE_FP32)}
