@@ -205,7 +205,10 @@ def main():
               "(multi-GPU slab decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
     if args.large:
-        print(json.dumps(large_rt(args.steps, args.warmup, args.fp64)))
+        out = large_rt(args.steps, args.warmup, args.fp64)
+        if args.calib_copy_mib > 0:
+            calib_copy(args.calib_copy_mib << 20)
+        print(json.dumps(out))
         return
 
     from gpesolver_amd import hip
