@@ -1764,13 +1764,50 @@ __global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_k) {
 enum { YR_FWD = 0, YR_INV = 1, YR_KIN = 2 };
 template <typename T, int L, int LPB, int DIR, int A>
 __device__ __forceinline__ void yline_reg(const cplx<T>* in, size_t istr, int nin, cplx<T>* out, size_t ostr, int nout,
-                                          bool valid, cplx<T>* E, const cplx<T>* tw, const cplx<T>* twl,
+                                          bool valid, T* E, const cplx<T>* tw, const cplx<T>* twl,
                                           const cplx<T>* ek) {
     constexpr int B = L / A, R = A / B, LS = LPB + 1;
     constexpr bool EX = DIR == YR_KIN;
     static_assert(L % A == 0 && A % B == 0 && LPB * B == 256, "y line split");
     const int ll = threadIdx.x % LPB, b = threadIdx.x / LPB;
     cplx<T> w[R][B];
+    // The exchange moves the real parts, then the imaginary parts, through one
+    // area of L (LPB + 1) reals: half the LDS of a complex area, so more blocks
+    // fit on a CU.  Slot (k1 B + bb) LS + ll holds element k1 B + bb of line ll.
+    // v -> w: thread b gives elements k1 B + b, takes rows k1 = b + B r.
+    auto v_to_w = [&](const cplx<T>(&v)[A]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h) __syncthreads();  // the real parts are read before the imaginary parts land
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = h ? v[k1].y : v[k1].x;
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int bb = 0; bb < B; ++bb) {
+                    const T e = E[((b + B * r) * B + bb) * LS + ll];
+                    if (h) w[r][bb].y = e; else w[r][bb].x = e;
+                }
+        }
+    };
+    // w -> v, the reverse
+    auto w_to_v = [&](cplx<T>(&v)[A]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h) __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int bb = 0; bb < B; ++bb) E[((b + B * r) * B + bb) * LS + ll] = h ? w[r][bb].y : w[r][bb].x;
+            __syncthreads();
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) {
+                const T e = E[(k1 * B + b) * LS + ll];
+                if (h) v[k1].y = e; else v[k1].x = e;
+            }
+        }
+    };
     if constexpr (DIR != YR_INV) {
         cplx<T> v[A];
 #pragma unroll
@@ -1786,16 +1823,9 @@ __device__ __forceinline__ void yline_reg(const cplx<T>* in, size_t istr, int ni
             else
                 v[k1] = gpe::cmul(v[k1], tw[(b * k1) % L]);
         }
+        v_to_w(v);
 #pragma unroll
-        for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = v[k1];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int k1 = b + B * r;
-#pragma unroll
-            for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * LS + ll];
-            gpe::ct::Dft<T, B, -1, EX>::run(w[r]);
-        }
+        for (int r = 0; r < R; ++r) gpe::ct::Dft<T, B, -1, EX>::run(w[r]);
         if constexpr (DIR == YR_FWD) {
             if (valid) {
 #pragma unroll
@@ -1828,13 +1858,9 @@ __device__ __forceinline__ void yline_reg(const cplx<T>* in, size_t istr, int ni
             else
                 w[r][bb] = gpe::cmulc(w[r][bb], tw[(bb * k1) % L]);
         }
-#pragma unroll
-        for (int bb = 0; bb < B; ++bb) E[(k1 * B + bb) * LS + ll] = w[r][bb];
     }
-    __syncthreads();
     cplx<T> v[A];
-#pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * LS + ll];
+    w_to_v(v);
     gpe::ct::Dft<T, A, +1, EX>::run(v);
     if (valid) {
 #pragma unroll
@@ -1848,8 +1874,10 @@ __device__ __forceinline__ void yline_reg(const cplx<T>* in, size_t istr, int ni
 template <int NY>
 struct PenY {
     static constexpr int L2 = 2 * NY, L1 = NY, A2 = 32, A1 = 16, LPB = 16;
-    static constexpr int EXA = L2 * (LPB + 1);  // exchange area (complex)
+    static constexpr int EXR = L2 * (LPB + 1);  // exchange area (reals)
     static constexpr int TAB = L2 > 3 * L1 ? L2 : 3 * L1;  // tables (complex)
+    template <typename T>
+    static constexpr size_t lds_bytes() { return EXR * sizeof(T) + TAB * sizeof(cplx<T>); }
 };
 
 // inverse y2 of S1 columns -> G (the k_pencil_ci step)
@@ -1857,8 +1885,8 @@ template <typename T, int NY>
 __global__ void __launch_bounds__(256) k_pencil_yci(PlaneArgs<T> a) {
     using Y = PenY<NY>;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
-    cplx<T>* TB = E + Y::EXA;
+    T* E = reinterpret_cast<T*>(lds_raw);
+    cplx<T>* TB = reinterpret_cast<cplx<T>*>(E + Y::EXR);
     const int nx = a.nx, nz = a.nz, ncb = (nz + Y::LPB - 1) / Y::LPB;
     const int i = blockIdx.x / ncb, c = (blockIdx.x - i * ncb) * Y::LPB + (int)threadIdx.x % Y::LPB;
     const bool valid = c < nz;
@@ -1877,8 +1905,8 @@ template <typename T, int NY>
 __global__ void __launch_bounds__(256) k_pencil_ycf(PlaneArgs<T> a) {
     using Y = PenY<NY>;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
-    cplx<T>* TB = E + Y::EXA;
+    T* E = reinterpret_cast<T*>(lds_raw);
+    cplx<T>* TB = reinterpret_cast<cplx<T>*>(E + Y::EXR);
     const int mode = a.mode;
     const int nx = a.nx, nz = a.nz, ncb = (nz + Y::LPB - 1) / Y::LPB;
     int blk = blockIdx.x;
@@ -2500,7 +2528,7 @@ static int launch_pencil(gpe_handle* h, const PlaneArgs<T>& a) {
     const bool yreg = kYReg && h->pen_yreg && ny == 256;
     using Y = PenY<256>;
     const int ncb = (nz + Y::LPB - 1) / Y::LPB;
-    const size_t ylds = (size_t)(Y::EXA + Y::TAB) * sizeof(cplx<T>);
+    const size_t ylds = Y::template lds_bytes<T>();
     if (ci) {
         if (yreg)
             hipExtLaunchKernelGGL((k_pencil_yci<T, 256>), dim3(nx * ncb), dim3(256), ylds, h->stream, e0, nullptr, 0,
