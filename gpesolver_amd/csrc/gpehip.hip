@@ -1753,7 +1753,7 @@ __global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_k) {
         xline_reg<T, X::LP, X::LPB_P, true, X::AP>(a, blockIdx.x - nblk_k, E, TB);
 }
 
-// Register y passes of the pencil plane pass (FP32, compiled ny): the
+// Register y passes of the pencil plane pass (compiled ny): the
 // four-step of the register x pass on y lines (kz columns of one plane, LPB
 // columns per 256-thread block), one direction at a time: YR_FWD forward
 // (inputs from index nin on are zero), YR_INV inverse (outputs from index
@@ -2523,8 +2523,8 @@ static int launch_pencil(gpe_handle* h, const PlaneArgs<T>& a) {
     const int nct = (nz + h->pen_c - 1) / h->pen_c, nrt = (ny + h->pen_r - 1) / h->pen_r;
     const bool ci = (a.mode & M_PINV) != 0;
     const bool cf = (a.mode & (M_RHO_FWD | M_KIN | M_NORM_OUT)) != 0;
-    // register y passes: FP32, compiled pencil shapes with ny = 256
-    constexpr bool kYReg = sizeof(T) == 4 && !std::is_same<Pol, PencilPolGeneric>::value;
+    // register y passes: compiled pencil shapes with ny = 256
+    constexpr bool kYReg = !std::is_same<Pol, PencilPolGeneric>::value;
     const bool yreg = kYReg && h->pen_yreg && ny == 256;
     using Y = PenY<256>;
     const int ncb = (nz + Y::LPB - 1) / Y::LPB;

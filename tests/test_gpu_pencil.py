@@ -129,8 +129,8 @@ def test_large_rt_first_steps_against_oracle(oracle, monkeypatch):
     orc.load_buffer(psi)
     orc.rt_steps(0, 3)
     g, gm = couplings(p, 0, 3)
-    # FP32 at ny = 256 runs the register y passes by default; "0" keeps the LDS column tiles covered
-    for prec, yreg in ((hip.GPE_FP64, "1"), (hip.GPE_FP32, "1"), (hip.GPE_FP32, "0")):
+    # ny = 256 runs the register y passes by default; "0" keeps the LDS column tiles covered
+    for prec, yreg in ((hip.GPE_FP64, "1"), (hip.GPE_FP64, "0"), (hip.GPE_FP32, "1"), (hip.GPE_FP32, "0")):
         monkeypatch.setenv("GPE_PENCIL_YREG", yreg)
         dev = dev_of(p, prec, V, psi)
         dev.rt_steps(3, g, gm)
@@ -143,8 +143,8 @@ def test_large_rt_first_steps_against_oracle(oracle, monkeypatch):
 
 def test_large_it_and_poisson_against_oracle(oracle, monkeypatch):
     """256x256x64: Poisson hook, 3 fused imaginary-time steps and the energies against the oracle,
-    FP64, FP32 with the register y passes (the default at ny = 256) and FP32 with the LDS column
-    tiles; the two FP32 column paths also against each other."""
+    FP64 and FP32, each with the register y passes (the default at ny = 256) and with the LDS
+    column tiles; the two column paths of each precision also against each other."""
     oracle.set_threads(min(16, os.cpu_count() or 1))
     p, op, V, psi = large_case(oracle)
     orc = oracle.Oracle(op)
@@ -155,7 +155,7 @@ def test_large_it_and_poisson_against_oracle(oracle, monkeypatch):
     orc.it_steps(3)
     eo = orc.energy()
     out = {}
-    for prec, yreg in ((hip.GPE_FP64, "1"), (hip.GPE_FP32, "1"), (hip.GPE_FP32, "0")):
+    for prec, yreg in ((hip.GPE_FP64, "1"), (hip.GPE_FP64, "0"), (hip.GPE_FP32, "1"), (hip.GPE_FP32, "0")):
         monkeypatch.setenv("GPE_PENCIL_YREG", yreg)
         dev = dev_of(p, prec, V, psi)
         dev.calc_fi3d()
@@ -172,6 +172,7 @@ def test_large_it_and_poisson_against_oracle(oracle, monkeypatch):
         assert r < (1e-10 if fp64 else 2e-6)
         assert de < (1e-10 if fp64 else 1e-6)
         dev.close()
+    assert rel(out[(hip.GPE_FP64, "1")], out[(hip.GPE_FP64, "0")]) < 1e-13
     assert rel(out[(hip.GPE_FP32, "1")], out[(hip.GPE_FP32, "0")]) < 1e-6
 
 
