@@ -52,49 +52,54 @@ def build(force: bool = False) -> str:
 _lib = None
 
 
+def bind(path: str):
+    """Load an oracle build at `path` and declare its C signatures."""
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.orc_solver_create.restype = vp
+    L.orc_solver_create.argtypes = [C.POINTER(OrcParams)]
+    L.orc_solver_destroy.argtypes = [vp]
+    L.orc_solver_params.restype = C.POINTER(OrcParams)
+    L.orc_solver_params.argtypes = [vp]
+    for name in ("orc_psi", "orc_psii", "orc_fi3d", "orc_pote", "orc_vdip_table", "orc_kinetic_table"):
+        getattr(L, name).restype = dp
+        getattr(L, name).argtypes = [vp]
+    L.orc_xnorma.restype = C.c_double
+    L.orc_xnorma.argtypes = [vp]
+    for name in ("orc_calc_fi3d", "orc_calc_norm", "orc_normalize", "orc_imag_iter_linear_step",
+                 "orc_imag_iter_nonlinear_step", "orc_real_fft_potential_half_step",
+                 "orc_real_fft_kinetic_step", "orc_imag_time_iter", "orc_real_time_iter",
+                 "orc_refresh_derived"):
+        getattr(L, name).argtypes = [vp]
+    L.orc_load_buffer.argtypes = [vp, dp]
+    L.orc_load_pote.argtypes = [vp, dp]
+    L.orc_calc_energy.argtypes = [vp, dp]
+    L.orc_it_steps.argtypes = [vp, C.c_long]
+    L.orc_rt_steps.argtypes = [vp, C.c_long, C.c_long]
+    L.orc_init_parameters.argtypes = [C.POINTER(OrcParams)]
+    L.orc_update_edd.argtypes = [C.POINTER(OrcParams), C.c_long]
+    L.orc_fill_potential.argtypes = [C.c_int, C.POINTER(OrcParams), dp]
+    L.orc_init_wavefunction.restype = C.c_int
+    L.orc_init_wavefunction.argtypes = [C.c_int, C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_double, dp]
+    L.orc_fft_1d.argtypes = [dp, C.c_int, C.c_int]
+    L.orc_fft_3d.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.restype = C.c_int
+    for name in ("orc_len_nm_to_au", "orc_len_au_to_nm", "orc_mass_Da_to_au", "orc_freq_Hz_to_au",
+                 "orc_ene_au_to_meV"):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [C.c_double]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        dp = C.POINTER(C.c_double)
-        vp = C.c_void_p
-        L.orc_solver_create.restype = vp
-        L.orc_solver_create.argtypes = [C.POINTER(OrcParams)]
-        L.orc_solver_destroy.argtypes = [vp]
-        L.orc_solver_params.restype = C.POINTER(OrcParams)
-        L.orc_solver_params.argtypes = [vp]
-        for name in ("orc_psi", "orc_psii", "orc_fi3d", "orc_pote", "orc_vdip_table", "orc_kinetic_table"):
-            getattr(L, name).restype = dp
-            getattr(L, name).argtypes = [vp]
-        L.orc_xnorma.restype = C.c_double
-        L.orc_xnorma.argtypes = [vp]
-        for name in ("orc_calc_fi3d", "orc_calc_norm", "orc_normalize", "orc_imag_iter_linear_step",
-                     "orc_imag_iter_nonlinear_step", "orc_real_fft_potential_half_step",
-                     "orc_real_fft_kinetic_step", "orc_imag_time_iter", "orc_real_time_iter",
-                     "orc_refresh_derived"):
-            getattr(L, name).argtypes = [vp]
-        L.orc_load_buffer.argtypes = [vp, dp]
-        L.orc_load_pote.argtypes = [vp, dp]
-        L.orc_calc_energy.argtypes = [vp, dp]
-        L.orc_it_steps.argtypes = [vp, C.c_long]
-        L.orc_rt_steps.argtypes = [vp, C.c_long, C.c_long]
-        L.orc_init_parameters.argtypes = [C.POINTER(OrcParams)]
-        L.orc_update_edd.argtypes = [C.POINTER(OrcParams), C.c_long]
-        L.orc_fill_potential.argtypes = [C.c_int, C.POINTER(OrcParams), dp]
-        L.orc_init_wavefunction.restype = C.c_int
-        L.orc_init_wavefunction.argtypes = [C.c_int, C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int,
-                                            C.c_int, C.c_double, dp]
-        L.orc_fft_1d.argtypes = [dp, C.c_int, C.c_int]
-        L.orc_fft_3d.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
-        L.orc_set_threads.argtypes = [C.c_int]
-        L.orc_get_threads.restype = C.c_int
-        for name in ("orc_len_nm_to_au", "orc_len_au_to_nm", "orc_mass_Da_to_au", "orc_freq_Hz_to_au",
-                     "orc_ene_au_to_meV"):
-            getattr(L, name).restype = C.c_double
-            getattr(L, name).argtypes = [C.c_double]
-        _lib = L
+        _lib = bind(_LIB_PATH)
     return _lib
 
 
@@ -159,8 +164,8 @@ def wavefunction(strategy: str, p: OrcParams, n_gauss_max=2, drops=(3, 3, 2), si
 class Oracle:
     """The oracle solver: one CpuGrossPitaevskiSolver state (cpu_solver.cpp)."""
 
-    def __init__(self, p: OrcParams):
-        self.L = lib()
+    def __init__(self, p: OrcParams, L=None):
+        self.L = lib() if L is None else L
         self.h = self.L.orc_solver_create(C.byref(p))
         self.shape = (p.nx, p.ny, p.nz)
 
