@@ -8,6 +8,12 @@ MULTIPLE_GAUSS(2) initial state with SplitMix64 phase noise (1e-3 rad, seed
 20261203), dt = 1e-4/omega_x.  FP32 bulk arithmetic, FP64 reductions.  The
 field, potential and all tables are resident in HBM before the timed region.
 
+Timing: after the warm-up, exactly --steps steps are timed between two
+device synchronisations, 5 times over; `value` is the median.  Between
+consecutive calls the solver keeps the last step's closing half phase pending
+and merges it into the next call's first step, so each timed call runs
+exactly --steps whole steps of work (one half phase carried in, one out).
+
 Prints ONE JSON line.  Besides steps/s it reports:
   roofline      the dominant kernel's achieved bandwidth (its algorithmic
                 bytes per launch / its average launch time, from start / stop
@@ -21,6 +27,8 @@ Prints ONE JSON line.  Besides steps/s it reports:
   cpu_baseline  the FP64 CPU oracle (the project's restatement of
                 cpu_solver + FFTW solvers) timed on this host's cores on a
                 bounded sample of the same workload;
+  rt_fp64       the same RT workload on the FP64 path (the reference's
+                precision), median steps/s and its step roofline;
   it_steps_per_s, it_time_to_converge
                 fused imaginary-time throughput on the same grid, and the
                 wall time of plain IT from MULTIPLE_GAUSS to |dE/N| < 1e-6
@@ -41,6 +49,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 GRID = (128, 128, 32)
+REPS = 5
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rt_128x128x32_fp32.json")
 
 
@@ -182,13 +191,33 @@ def large_rt(steps: int, warmup: int, fp64: bool) -> dict:
     return out
 
 
+def timed_rt(dev, steps: int, reps: int = REPS) -> list:
+    """Seconds of `reps` timed calls of exactly `steps` RT steps each."""
+    out = []
+    for _ in range(reps):
+        dev.synchronize()
+        t0 = time.perf_counter()
+        dev.rt_steps(steps)
+        dev.synchronize()
+        out.append(time.perf_counter() - t0)
+    return out
+
+
+def host_cores() -> dict:
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"cpu_count": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--fp64", action="store_true", help="also time the FP64 path")
+    ap.add_argument("--fp64", action="store_true", help="--large: also time the FP64 path")
     ap.add_argument("--it-max-steps", type=int, default=20000, help="cap of the IT time-to-convergence leg")
     ap.add_argument("--rt-only", action="store_true",
                     help="profiling runs: the RT leg only, on the default launch shapes (split plane pass, "
@@ -220,11 +249,9 @@ def main():
     # warm-up
     dev.rt_steps(args.warmup)
     dev.synchronize()
-    # timed region: exactly K steps
-    t0 = time.perf_counter()
-    dev.rt_steps(args.steps)
-    dev.synchronize()
-    el = time.perf_counter() - t0
+    # timed region: exactly K steps, REPS times; the median
+    reps = timed_rt(dev, args.steps)
+    el = float(np.median(reps))
     steps_per_s = args.steps / el
 
     # per-kernel durations from HIP events on the solver stream (separate pass)
@@ -267,16 +294,26 @@ def main():
     it_conv_s = time.perf_counter() - t1
     dc.close()
 
-    extra = {}
-    if args.fp64:
-        _, d64, _, _ = build_case(hip.GPE_FP64)
-        d64.rt_steps(args.warmup)
-        d64.synchronize()
-        t2 = time.perf_counter()
-        d64.rt_steps(args.steps)
-        d64.synchronize()
-        extra["fp64_rt_steps_per_s"] = args.steps / (time.perf_counter() - t2)
-        d64.close()
+    # the same RT workload on the FP64 path
+    _, d64, _, _ = build_case(hip.GPE_FP64)
+    tuned64 = d64.autotune(100)
+    d64.rt_steps(args.warmup)
+    d64.synchronize()
+    reps64 = timed_rt(d64, args.steps)
+    el64 = float(np.median(reps64))
+    d64.set_timing(True)
+    d64.rt_steps(min(args.steps * 10, 500))
+    tm64 = d64.timing()
+    d64.set_timing(False)
+    d64.close()
+    mb64 = model_bytes(nx, ny, nz, fp64=True)
+    extra = {"rt_fp64": {"steps_per_s": args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps,
+                         "rep_ms": [1e3 * r for r in reps64],
+                         "avg_launch_us": {"plane": tm64["plane_us"], "line": tm64["line_us"]},
+                         "tuned": {k: tuned64[k] for k in ("split_plane_pass", "register_x_pass", "graph_replay")},
+                         "step_roofline": {"model_bytes_per_step": mb64,
+                                           "achieved_GBps": mb64 * args.steps / el64 / 1e9,
+                                           "frac": mb64 * args.steps / el64 / HBM_PEAK}}}
 
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib << 20)
@@ -290,6 +327,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": 1e3 * el / args.steps,
+        "rep_ms": [1e3 * r for r in reps],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -330,9 +368,14 @@ def main():
                                 "start": "MULTIPLE_GAUSS(2), 4-cell widths, 1e-3 rad phase noise"},
         **extra,
     }
+    hc = host_cores()
+    out["host"] = hc
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the GPU box allots 16 host CPUs to a one-GPU job (OMP_NUM_THREADS=16
+        # there); use what the process may run on, up to that share
+        threads = max(1, min(16, hc["affinity"]))
         out["cpu_baseline"] = cpu_baseline(threads)
+        out["cpu_baseline"]["host_cpu_count"] = hc["cpu_count"]
     print(json.dumps(out), flush=True)
 
 

@@ -2192,6 +2192,19 @@ struct gpe_handle {
     // graph-replayed steady state (chunks of kGraphSteps steps)
     double* coef = nullptr;      // per-step coupling table (2 doubles per step)
     size_t coef_cap = 0;         // steps the table holds
+    double* coef_host = nullptr; // pinned staging of the table (coef_cap + 2 pairs)
+    hipEvent_t coef_ev = nullptr; // recorded after the last upload from coef_host
+    bool coef_const = false;     // the table holds coef_n copies of coef_cval
+    double coef_cval[2] = {0, 0};
+    size_t coef_n = 0;
+    // Real time: the closing half phase of the last step of a call is left
+    // pending and merged into the first step of the next call (the two half
+    // steps between consecutive steps share Phi and the density).  The state
+    // is then the x-transformed Poisson spectrum and kinetic array of parity
+    // rt_par; materialize() applies the half phase and writes psi and fi3d.
+    bool rt_pending = false;
+    int rt_par = 0;
+    double rt_pend[2] = {0, 0};  // scaled couplings (g', gamma') of that half phase
     int* cbase = nullptr;        // device step counter read by graph nodes
     hipGraphExec_t rt_graph = nullptr;
     hipGraphExec_t it_graph[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -2620,8 +2633,40 @@ static int launch_line(gpe_handle* h, bool poisson, bool kinetic, void* s1 = nul
     return launch_line_pol<T, LinePolGeneric>(h, a, nblk);
 }
 
+// S1 / Tk buffers of step parity par: the split plane pass reads pair par
+// and writes pair 1 - par (the unsplit pass works in place on s1 / tk).
+// Every loop starts on parity 0 (its prologue writes s1).
+static void* rt_s1(gpe_handle* h, int par) { return (h->plane_split && par) ? h->s1b : h->s1; }
+static void* rt_tk(gpe_handle* h, int par) { return (h->plane_split && par) ? h->tkb : h->tk; }
+
+static int ensure_split_buffers(gpe_handle* h, bool tk) {
+    if (!h->plane_split) return 0;
+    const size_t cs = csize(h);
+    if (!h->s1b) GPE_CHECK(hipMalloc(&h->s1b, (size_t)2 * h->p.ny * h->p.nx * h->p.nz * cs));
+    if (tk && !h->tkb) GPE_CHECK(hipMalloc(&h->tkb, h->N * cs));
+    return 0;
+}
+
+// Apply a pending real-time half phase: finish the Poisson solve of the last
+// step's Phi, rotate psi by exp(-i dt/2 Veff) and write psi and fi3d.
+template <typename T>
+static int flush_rt(gpe_handle* h) {
+    if (!h->rt_pending) return 0;
+    h->rt_pending = false;
+    PlaneArgs<T> a = plane_args<T>(h, M_PINV | M_PHI_STORE | M_IN_TK | M_PHASE_HALF | M_OUT);
+    a.s1 = a.s1o = (cplx<T>*)rt_s1(h, h->rt_par);
+    a.tk = a.tko = (cplx<T>*)rt_tk(h, h->rt_par);
+    a.out = (cplx<T>*)h->store[h->cur];
+    a.pg = h->rt_pend[0];
+    a.pgm = h->rt_pend[1];
+    return launch_plane<T>(h, a);
+}
+
+// Bring the state to the reference's form: psi in store[cur] (normalised),
+// fi3d in phi.
 template <typename T>
 static int materialize(gpe_handle* h) {
+    if (flush_rt<T>(h)) return 1;
     if (!h->norm_pending) return 0;
     hipLaunchKernelGGL((k_scale<T>), dim3(1024), dim3(256), 0, h->stream, (cplx<T>*)h->store[h->cur], h->N,
                        h->part[h->pcur], h->p.nx);
@@ -2649,6 +2694,7 @@ template <typename T>
 static int do_normalize(gpe_handle* h) {
     // normalize() divides by sqrt of the norm computed by the last calc_norm
     // (part[pcur]); the IT loop leaves the partials of the new state there.
+    if (flush_rt<T>(h)) return 1;
     hipLaunchKernelGGL((k_scale<T>), dim3(1024), dim3(256), 0, h->stream, (cplx<T>*)h->store[h->cur], h->N,
                        h->part[h->pcur], h->p.nx);
     GPE_CHECK(hipGetLastError());
@@ -2740,19 +2786,35 @@ static int finish_timing(gpe_handle* h, size_t nev) {
 constexpr int kGraphSteps = 64;  // steady-state steps per captured graph (even)
 static_assert(kGraphSteps % 2 == 0, "a replay must end on the S1 / Tk parity it started on");
 
-static int ensure_coef(gpe_handle* h, size_t steps) {
-    if (steps <= h->coef_cap) return 0;
+// Coupling table of `pairs` (g', gamma') pairs on the device, with its pinned
+// host staging buffer.
+static int ensure_coef(gpe_handle* h, size_t pairs) {
+    if (pairs <= h->coef_cap) return 0;
     if (h->rt_graph) {
         (void)hipGraphExecDestroy(h->rt_graph);
         h->rt_graph = nullptr;
     }
+    if (h->coef_ev) GPE_CHECK(hipEventSynchronize(h->coef_ev));
     if (h->coef) GPE_CHECK(hipFree(h->coef));
+    if (h->coef_host) GPE_CHECK(hipHostFree(h->coef_host));
     h->coef = nullptr;
-    const size_t cap = std::max<size_t>(steps, 4096);
-    // one spare step: the plane pass loads a step's two coupling pairs at once
-    GPE_CHECK(hipMalloc(&h->coef, (cap + 1) * 2 * sizeof(double)));
+    h->coef_host = nullptr;
+    h->coef_cap = 0;
+    h->coef_const = false;
+    const size_t cap = std::max<size_t>(pairs, 4096);
+    GPE_CHECK(hipMalloc(&h->coef, cap * 2 * sizeof(double)));
+    GPE_CHECK(hipHostMalloc((void**)&h->coef_host, cap * 2 * sizeof(double), hipHostMallocDefault));
+    if (!h->coef_ev) GPE_CHECK(hipEventCreateWithFlags(&h->coef_ev, hipEventDisableTiming));
     h->coef_cap = cap;
     return 0;
+}
+
+// Scaled couplings (phi units) of one RT step with couplings g, gamma.
+static void rt_coupling(const gpe_handle* h, double g, double gm, double* out) {
+    const gpe_params& p = h->p;
+    const double dV = p.dx * p.dy * p.dz;
+    out[0] = (g - p.cdd / 3) * p.n_atoms / dV;
+    out[1] = gm * p.w_15 / pow(dV, 1.5);
 }
 
 // Capture `body` (launches on h->stream) into an executable graph.
@@ -2770,20 +2832,6 @@ static int capture(gpe_handle* h, hipGraphExec_t* out, F&& body) {
     e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(h, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    return 0;
-}
-
-// S1 / Tk buffers of step parity par: the split plane pass reads pair par
-// and writes pair 1 - par (the unsplit pass works in place on s1 / tk).
-// Every loop starts on parity 0 (its prologue writes s1).
-static void* rt_s1(gpe_handle* h, int par) { return (h->plane_split && par) ? h->s1b : h->s1; }
-static void* rt_tk(gpe_handle* h, int par) { return (h->plane_split && par) ? h->tkb : h->tk; }
-
-static int ensure_split_buffers(gpe_handle* h, bool tk) {
-    if (!h->plane_split) return 0;
-    const size_t cs = csize(h);
-    if (!h->s1b) GPE_CHECK(hipMalloc(&h->s1b, (size_t)2 * h->p.ny * h->p.nx * h->p.nz * cs));
-    if (tk && !h->tkb) GPE_CHECK(hipMalloc(&h->tkb, h->N * cs));
     return 0;
 }
 
@@ -2821,6 +2869,7 @@ static int it_step_launch(gpe_handle* h, bool last, int& par) {
 template <typename T>
 static int do_it_iters(gpe_handle* h, long n) {
     if (n <= 0) return 0;
+    if (flush_rt<T>(h)) return 1;
     if (ensure_split_buffers(h, false)) return 1;
     int par = 0;
     // prologue: partials of the current state + Poisson start of |store|^2
@@ -2873,7 +2922,10 @@ static int do_it_iters(gpe_handle* h, long n) {
     return finish_timing(h, ne);
 }
 
-// Steady-state RT plane pass for step t -> t+1 (coupling from the table).
+// Steady-state RT plane pass merging the closing half phase of one step with
+// the opening half phase of the next (coupling pairs cidx, cidx + 1 of the
+// table, offset by *cbase when set); reads S1 / Tk of parity par and writes
+// parity 1 - par when split.
 template <typename T>
 static PlaneArgs<T> rt_steady_args(gpe_handle* h, int cidx, const int* cbase, int par) {
     PlaneArgs<T> a = plane_args<T>(h, M_PINV | M_IN_TK | M_PHASE_FULL | M_RHO_FWD | M_KIN);
@@ -2881,88 +2933,133 @@ static PlaneArgs<T> rt_steady_args(gpe_handle* h, int cidx, const int* cbase, in
     a.coef = h->coef;
     a.cbase = cbase;
     a.cidx = cidx;
+    a.s1 = a.s1o = (cplx<T>*)rt_s1(h, par);
+    a.tk = a.tko = (cplx<T>*)rt_tk(h, par);
     if (h->plane_split) {
         a.split = 1;
-        a.s1 = (cplx<T>*)rt_s1(h, par);
-        a.tk = (cplx<T>*)rt_tk(h, par);
         a.s1o = (cplx<T>*)rt_s1(h, 1 - par);
         a.tko = (cplx<T>*)rt_tk(h, 1 - par);
     }
     return a;
 }
 
+// One pipelined RT step: the steady plane pass, then the x pass of the new
+// Poisson spectrum and kinetic array (parity 1 - par when split).
+template <typename T>
+static int rt_step_launch(gpe_handle* h, int cidx, const int* cbase, int& par) {
+    if (launch_plane<T>(h, rt_steady_args<T>(h, cidx, cbase, par))) return 1;
+    const int np = h->plane_split ? 1 - par : par;
+    if (launch_line<T>(h, true, true, rt_s1(h, np), rt_tk(h, np))) return 1;
+    par = np;
+    return 0;
+}
+
+__global__ void k_set_int(int* p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// n real_time_iter steps (solver.cpp:107-112), with per-step couplings g, gm
+// (the update_edd ramp; null: the current couplings).  Pipelined: step t's
+// plane pass applies the closing half phase of step t-1 and the opening half
+// phase of step t as one phase (both use Phi of step t-1: real_time_iter
+// computes Phi once, after its first half step), starts Phi of step t and runs
+// the z / y kinetic propagation; the x pass then does the x parts of both.
+// The closing half phase of the last step is left pending (flush_rt).
 template <typename T>
 static int do_rt_iters(gpe_handle* h, long n, const double* g, const double* gm) {
     if (n <= 0) return 0;
-    if (materialize<T>(h)) return 1;
-    const gpe_params& p = h->p;
-    const double dV = p.dx * p.dy * p.dz;
-    // per-step scaled couplings (update_edd ramp or constant)
-    if (ensure_coef(h, (size_t)n)) return 1;
-    std::vector<double> cf(2 * (size_t)n);
-    for (long t = 0; t < n; ++t) {
-        cf[2 * t] = ((g ? g[t] : h->ggp11) - p.cdd / 3) * p.n_atoms / dV;
-        cf[2 * t + 1] = (gm ? gm[t] : h->gamma) * p.w_15 / pow(dV, 1.5);
+    if (h->norm_pending && materialize<T>(h)) return 1;  // (rt_pending and norm_pending never hold together)
+    const bool pend = h->rt_pending;
+    const size_t off = pend ? 1 : 0;
+    // coupling table: [pending pair], pairs of steps 0..n-1, one spare (a
+    // steady pass loads two pairs)
+    const size_t pairs = (size_t)n + off + 1;
+    if (ensure_coef(h, pairs)) return 1;
+    const bool constant = !g && !gm;
+    double cc[2];
+    rt_coupling(h, h->ggp11, h->gamma, cc);
+    const bool pend_same = !pend || (h->rt_pend[0] == cc[0] && h->rt_pend[1] == cc[1]);
+    const bool cached = constant && pend_same && h->coef_const && h->coef_cval[0] == cc[0] &&
+                        h->coef_cval[1] == cc[1] && h->coef_n >= pairs;
+    if (!cached) {
+        // the previous upload from the staging buffer must be complete
+        GPE_CHECK(hipEventSynchronize(h->coef_ev));
+        double* cf = h->coef_host;
+        const size_t fill = constant ? h->coef_cap : pairs;
+        for (size_t t = 0; t < fill; ++t) {
+            const long s = std::min<long>((long)t - (long)off, n - 1);
+            if (constant || s < 0) {
+                cf[2 * t] = cc[0];
+                cf[2 * t + 1] = cc[1];
+            } else {
+                rt_coupling(h, g ? g[s] : h->ggp11, gm ? gm[s] : h->gamma, cf + 2 * t);
+            }
+        }
+        if (pend) {
+            cf[0] = h->rt_pend[0];
+            cf[1] = h->rt_pend[1];
+        }
+        GPE_CHECK(hipMemcpyAsync(h->coef, cf, fill * 2 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        GPE_CHECK(hipEventRecord(h->coef_ev, h->stream));
+        h->coef_const = constant && pend_same;
+        h->coef_cval[0] = cc[0];
+        h->coef_cval[1] = cc[1];
+        h->coef_n = fill;
     }
-    GPE_CHECK(hipMemcpyAsync(h->coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (ensure_split_buffers(h, true)) return 1;
-    // prologue: first half phase with the current fi3d, Poisson start, z/y kinetic
-    PlaneArgs<T> a = plane_args<T>(h, M_PHI_LOAD | M_PHASE_HALF | M_RHO_FWD | M_KIN);
-    a.in = (const cplx<T>*)h->store[h->cur];
-    a.coef = h->coef;
-    a.cidx = 0;
-    if (launch_plane<T>(h, a)) return 1;
     size_t ne = 0;
     ev_record(h, ne++);
-    long t = 0;
-    const long steady = n - 1;
-    if (h->use_graphs && !h->timing && steady >= kGraphSteps) {
+    int par = pend ? h->rt_par : 0;
+    long t = 0;  // steps launched
+    if (!pend) {
+        // opening half phase of step 0 with the current fi3d, Poisson start,
+        // z / y kinetic, then its x pass
+        PlaneArgs<T> a = plane_args<T>(h, M_PHI_LOAD | M_PHASE_HALF | M_RHO_FWD | M_KIN);
+        a.in = (const cplx<T>*)h->store[h->cur];
+        a.s1 = a.s1o = (cplx<T>*)rt_s1(h, 0);
+        a.tk = a.tko = (cplx<T>*)rt_tk(h, 0);
+        a.coef = h->coef;
+        a.cidx = 0;
+        if (launch_plane<T>(h, a)) return 1;
+        if (launch_line<T>(h, true, true, rt_s1(h, 0), rt_tk(h, 0))) return 1;
+        t = 1;
+    }
+    // step t merges pairs (t - 1 + off, t + off)
+    if (h->use_graphs && !h->timing && n - t >= kGraphSteps + (par ? 1 : 0)) {
+        if (par) {  // the captured chunk starts on parity 0
+            if (rt_step_launch<T>(h, (int)(t - 1 + off), nullptr, par)) return 1;
+            ++t;
+        }
         if (!h->rt_graph) {
             // kGraphSteps is even: a replay ends on the buffer parity it started on
             const int rc = capture(h, &h->rt_graph, [&]() -> int {
-                for (int s = 0; s < kGraphSteps; ++s) {
-                    const int par = s & 1;
-                    if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
-                    if (launch_plane<T>(h, rt_steady_args<T>(h, s, h->cbase, par))) return 1;
-                }
+                int cpar = 0;
+                for (int s = 0; s < kGraphSteps; ++s)
+                    if (rt_step_launch<T>(h, s, h->cbase, cpar)) return 1;
                 hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, h->stream, h->cbase, kGraphSteps);
                 GPE_CHECK(hipGetLastError());
                 return 0;
             });
             if (rc) return 1;
         }
-        GPE_CHECK(hipMemsetAsync(h->cbase, 0, sizeof(int), h->stream));
-        while (steady - t >= kGraphSteps) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->cbase, (int)(t - 1 + off));
+        GPE_CHECK(hipGetLastError());
+        while (n - t >= kGraphSteps) {
             GPE_CHECK(hipGraphLaunch(h->rt_graph, h->stream));
             t += kGraphSteps;
         }
     }
     h->kn[0] = h->kn[1] = 0;
     h->tim_on = true;  // time the steady-state launches
-    for (; t < steady; ++t) {
-        const int par = (int)(t & 1);
-        if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
-        ev_record(h, ne++);
-        if (launch_plane<T>(h, rt_steady_args<T>(h, (int)t, nullptr, par))) return 1;
-        ev_record(h, ne++);
-    }
+    for (; t < n; ++t)
+        if (rt_step_launch<T>(h, (int)(t - 1 + off), nullptr, par)) return 1;
     h->tim_on = false;
-    // last step: finish with a half phase and write psi, fi3d
-    const int par = (int)(steady & 1);
-    if (launch_line<T>(h, true, true, rt_s1(h, par), rt_tk(h, par))) return 1;
     ev_record(h, ne++);
-    a = plane_args<T>(h, M_PINV | M_PHI_STORE | M_IN_TK | M_PHASE_HALF | M_OUT);
-    a.s1 = a.s1o = (cplx<T>*)rt_s1(h, par);
-    a.tk = a.tko = (cplx<T>*)rt_tk(h, par);
-    a.out = (cplx<T>*)h->store[h->cur];
-    a.coef = h->coef;
-    a.cidx = (int)(n - 1);
-    if (launch_plane<T>(h, a)) return 1;
-    ev_record(h, ne++);
+    h->rt_pending = true;
+    h->rt_par = par;
+    rt_coupling(h, g ? g[n - 1] : h->ggp11, gm ? gm[n - 1] : h->gamma, h->rt_pend);
     if (g) h->ggp11 = g[n - 1];
     if (gm) h->gamma = gm[n - 1];
-    // the host-side coupling vector must outlive the async copy
-    GPE_CHECK(hipStreamSynchronize(h->stream));
     return finish_timing(h, ne);
 }
 
@@ -3449,6 +3546,8 @@ void gpe_destroy(gpe_handle* h) {
     for (hipGraphExec_t g : h->it_graph)
         if (g) (void)hipGraphExecDestroy(g);
     if (h->coef) (void)hipFree(h->coef);
+    if (h->coef_host) (void)hipHostFree(h->coef_host);
+    if (h->coef_ev) (void)hipEventDestroy(h->coef_ev);
     if (h->cbase) (void)hipFree(h->cbase);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -3460,6 +3559,7 @@ int gpe_load_buffer(gpe_handle* h, const double* psi) {
     if (upload_real(h, h->store[h->cur], psi, n2, h->sqdv)) return 1;
     if (upload_real(h, h->psii, psi, n2, h->sqdv)) return 1;
     h->norm_pending = false;
+    h->rt_pending = false;  // the loaded state replaces a pending RT step
     // calc_norm + normalize (solver.cpp:118-125)
     if (DISPATCH(h, do_calc_norm, h, nullptr)) return 1;
     return DISPATCH(h, do_normalize, h);
@@ -3467,6 +3567,7 @@ int gpe_load_buffer(gpe_handle* h, const double* psi) {
 
 int gpe_load_pote(gpe_handle* h, const double* V) {
     if (!h || !V) return fail(h, "null argument");
+    if (DISPATCH(h, flush_rt, h)) return 1;  // a pending half phase uses the old V
     return upload_real(h, h->V, V, h->N, 1.0);
 }
 
@@ -3478,11 +3579,13 @@ int gpe_export_psi(gpe_handle* h, double* psi) {
 
 int gpe_export_fi3d(gpe_handle* h, double* fi3d) {
     if (!h || !fi3d) return fail(h, "null argument");
+    if (DISPATCH(h, flush_rt, h)) return 1;
     return download_real(h, fi3d, h->phi, h->N, 1.0);
 }
 
 int gpe_import_fi3d(gpe_handle* h, const double* fi3d) {
     if (!h || !fi3d) return fail(h, "null argument");
+    if (DISPATCH(h, flush_rt, h)) return 1;
     return upload_real(h, h->phi, fi3d, h->N, 1.0);
 }
 
@@ -3666,6 +3769,8 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
         for (int sp = s->pencil ? 0 : 1; sp >= 0 && !rc; --sp)
             for (int lr = reg_ok ? 1 : 0; lr >= 0 && !rc; --lr)
                 for (int gr = graphs_ok ? 1 : 0; gr >= 0 && !rc; --gr) {
+                    rc = DISPATCH(s, materialize, s);
+                    if (rc) break;
                     s->plane_split = sp != 0;
                     s->line_reg = lr != 0;
                     s->use_graphs = gr != 0;
@@ -3688,6 +3793,8 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune[key] = best;
     }
+    // a pending RT state lives in the buffers of the current variant
+    if (DISPATCH(h, materialize, h)) return 1;
     if (h->plane_split != best.split || h->line_reg != best.line_reg) drop_graphs(h);
     h->plane_split = best.split;
     h->line_reg = best.line_reg;
