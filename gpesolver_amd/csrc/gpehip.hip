@@ -2034,9 +2034,11 @@ __global__ void k_potential_half(cplx<T>* psi, const T* V, const T* phi, int nx,
 template <typename T, int NT>
 __global__ void __launch_bounds__(NT) k_energy(const cplx<T>* psi, const T* V, const T* phi, int nx, int ny, int nz,
                                                double ix2, double iy2, double iz2, double m, double cdd, double gq,
-                                               double gmq, double mu, int pass, double* out) {
+                                               double gmq, double mu, int pass, double* out,
+                                               const double* mu_dev = nullptr) {
     __shared__ double red[NT / 64 + 1];
     const int i = blockIdx.x;
+    if (mu_dev) mu = *mu_dev;
     const int npl = ny * nz;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (i >= 1 && i <= nx - 2) {
@@ -2088,6 +2090,45 @@ __global__ void __launch_bounds__(NT) k_energy(const cplx<T>* psi, const T* V, c
         if (threadIdx.x == 0) out[(size_t)i * 8 + (pass == 0 ? t : 6)] = s;
     }
 }
+
+// Energies from the per-plane partials of k_energy, on the device: pass 0
+// adds the six pass-0 sums in plane order into sums[0..5] (mu = sums[5]);
+// pass 1 adds the residual sums and writes the eight outputs (gpe_energies
+// order) with the formulas and operation order of do_energy.
+__global__ void k_energy_sums(const double* part, int nx, int pass, double* sums, double* out, double m, double N,
+                              double dV, double g, double gamma, double cdd) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (pass == 0) {
+        double s[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < nx; ++i)
+            for (int t = 0; t < 6; ++t) s[t] += part[(size_t)i * 8 + t];
+        for (int t = 0; t < 6; ++t) sums[t] = s[t];
+        return;
+    }
+    double r2 = 0;
+    for (int i = 0; i < nx; ++i) r2 += part[(size_t)i * 8 + 6];
+    const double* s = sums;
+    const double e_kin = s[0] / (2 * m) * N;
+    const double e_pot = s[1] * N;
+    const double e_int = 0.5 * g * s[2] / dV * N * N;
+    const double e_ext = 0.5 * cdd * s[3] * N - cdd / 3. * s[2] / dV / 2. * N * N;
+    const double e_bmf = 2. / 5. * gamma * s[4] / pow(dV, 1.5) * pow(N, 2.5);
+    out[0] = e_kin;
+    out[1] = e_pot;
+    out[2] = e_int;
+    out[3] = e_ext;
+    out[4] = e_bmf;
+    out[5] = e_kin + e_pot + e_int + e_ext + e_bmf;
+    out[6] = s[5];
+    out[7] = sqrt(r2);
+}
+
+// Scratch of gpe_calc_energy_noalloc: per-plane partials (8 doubles per
+// plane) and the pass-0 sums.  Host code never uses this name as an address:
+// it takes the per-device address from hipGetSymbolAddress (energy_scratch).
+constexpr int kNoAllocPlanes = 4096;
+static __device__ __align__(256) double scratch[kNoAllocPlanes * 8 + 8];
 
 // Unfused IT step (the two hooks imag_iter_linear_step + imag_iter_nonlinear_step):
 // out interior from in (already normalised), out boundary from psii.
@@ -2144,6 +2185,7 @@ __global__ void k_it_stencil(const cplx<T>* in, cplx<T>* out, const cplx<T>* psi
 struct gpe_handle {
     gpe_params p;
     int prec;
+    int dev = 0;  // the device current at gpe_create
     size_t N;
     hipStream_t stream = nullptr;
     bool own_stream = true;
@@ -2223,6 +2265,26 @@ struct gpe_handle {
 };
 
 static std::string g_create_err;
+
+// Device address of `scratch` per device, set once per device at gpe_create
+// (with that device current): each device holds its own copy of the array.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxDevices] = {};
+static hipError_t init_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch[dev] = static_cast<double*>(ptr);
+    return e;
+}
+static double* energy_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[dev];
+}
 
 static int fail(gpe_handle* h, const std::string& msg) {
     if (h)
@@ -3104,6 +3166,34 @@ static int do_energy(gpe_handle* h, gpe_energies* out) {
 }
 
 
+// gpe_calc_energy without allocation, host copies or synchronisation: the
+// partials and sums live in the device's `scratch`, the outputs go to out_dev.
+template <typename T>
+static int do_energy_noalloc(gpe_handle* h, double* out_dev) {
+    const gpe_params& p = h->p;
+    if (p.nx > kNoAllocPlanes)
+        return fail(h, "gpe_calc_energy_noalloc: nx = " + std::to_string(p.nx) + " exceeds its scratch (" +
+                           std::to_string(kNoAllocPlanes) + " planes)");
+    double* sc = energy_scratch(h->dev);
+    if (!sc) return fail(h, "gpe_calc_energy_noalloc: no scratch address for this device");
+    if (materialize<T>(h)) return 1;
+    const double dV = p.dx * p.dy * p.dz;
+    const double gq = (h->ggp11 - p.cdd / 3) * p.n_atoms / dV, gmq = h->gamma * p.w_15 / pow(dV, 1.5);
+    double* part = sc;
+    double* sums = sc + (size_t)kNoAllocPlanes * 8;
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL((k_energy<T, 256>), dim3(p.nx), dim3(256), 0, h->stream, (const cplx<T>*)h->store[h->cur],
+                           (const T*)h->V, (const T*)h->phi, p.nx, p.ny, p.nz, 1.0 / (p.dx * p.dx),
+                           1.0 / (p.dy * p.dy), 1.0 / (p.dz * p.dz), p.m, p.cdd, gq, gmq, 0.0, pass, part,
+                           pass ? sums + 5 : nullptr);
+        GPE_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_energy_sums, dim3(1), dim3(64), 0, h->stream, part, p.nx, pass, sums, out_dev, p.m,
+                           p.n_atoms, dV, h->ggp11, h->gamma, p.cdd);
+        GPE_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
 // ===========================================================================
 // Batch: many instances (grids may differ) stepped together, one launch per
 // pass over all active instances, steady-state chunks replayed as hipGraphs.
@@ -3473,6 +3563,11 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
         gpe_destroy(h);
         return 1;
     };
+    if (hipGetDevice(&h->dev) != hipSuccess) return bad("hipGetDevice failed");
+    {
+        const hipError_t e = init_scratch(h->dev);
+        if (e != hipSuccess) return bad(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bad("stream create failed");
     const size_t cs = csize(h), rs = rsize(h);
     const size_t n = h->N;
@@ -3605,6 +3700,10 @@ int gpe_real_kinetic_step(gpe_handle* h) { return h ? DISPATCH(h, do_kinetic, h)
 int gpe_calc_energy(gpe_handle* h, gpe_energies* out) {
     if (!h || !out) return fail(h, "null argument");
     return DISPATCH(h, do_energy, h, out);
+}
+int gpe_calc_energy_noalloc(gpe_handle* h, double* out_dev) {
+    if (!h || !out_dev) return fail(h, "null argument");
+    return DISPATCH(h, do_energy_noalloc, h, out_dev);
 }
 int gpe_imag_time_iters(gpe_handle* h, long n) { return h ? DISPATCH(h, do_it_iters, h, n) : 1; }
 int gpe_imag_time_until(gpe_handle* h, long max_steps, long check_every, double tol, long* steps_out,

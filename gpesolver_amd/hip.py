@@ -21,7 +21,7 @@ EXPORTS = (
     "gpe_create", "gpe_destroy", "gpe_last_error", "gpe_version",
     "gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d",
     "gpe_set_coupling", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
-    "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy",
+    "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy", "gpe_calc_energy_noalloc",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_imag_time_until", "gpe_synchronize", "gpe_set_timing",
     "gpe_get_timing", "gpe_autotune",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
@@ -87,6 +87,8 @@ def load(path: str = LIB_PATH):
     L.gpe_calc_norm.restype = ip
     L.gpe_calc_energy.argtypes = [vp, C.POINTER(GpeEnergies)]
     L.gpe_calc_energy.restype = ip
+    L.gpe_calc_energy_noalloc.argtypes = [vp, vp]
+    L.gpe_calc_energy_noalloc.restype = ip
     L.gpe_imag_time_iters.argtypes = [vp, C.c_long]
     L.gpe_imag_time_iters.restype = ip
     L.gpe_imag_time_until.argtypes = [vp, C.c_long, C.c_long, C.c_double, C.POINTER(C.c_long), dp]
@@ -211,6 +213,11 @@ class Device:
         e = GpeEnergies()
         self._check(self.L.gpe_calc_energy(self.h, C.byref(e)))
         return e.as_dict()
+
+    def energy_noalloc(self, out_dev: int):
+        """gpe_calc_energy_noalloc: the eight energies to device memory at
+        out_dev (8 doubles), enqueued on the handle's stream."""
+        self._check(self.L.gpe_calc_energy_noalloc(self.h, C.c_void_p(out_dev)))
 
     # loops
     def it_steps(self, n: int):

@@ -86,6 +86,14 @@ int gpe_imag_iter_step(gpe_handle *h);              /* imag_iter_linear_step + i
 int gpe_real_potential_half_step(gpe_handle *h);    /* cpu_solver.cpp:99-131 */
 int gpe_real_kinetic_step(gpe_handle *h);           /* fftw_split_solver.cpp:44-86 */
 int gpe_calc_energy(gpe_handle *h, gpe_energies *out); /* cpu_solver.cpp:137-194 (+ mu, residual) */
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under hipGraph capture, tight serving loops): gpe_calc_energy with no
+ * allocation, host copy or synchronisation.  The eight values are written,
+ * in gpe_energies order, to out_dev (8 doubles of device memory) on the
+ * handle's stream; the per-plane partials live in a static device array of
+ * the library (4096 planes at most: larger nx is rejected).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU race. */
+int gpe_calc_energy_noalloc(gpe_handle *h, double *out_dev);
 
 /* Fused loops: n consecutive imag_time_iter (src/solver/solver.cpp:99-105) /
  * real_time_iter (src/solver/solver.cpp:107-112) calls.  For the real-time
