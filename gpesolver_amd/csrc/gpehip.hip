@@ -171,11 +171,12 @@ struct LineArgs {
     LinePlan px2, px;
 };
 
-// rho^(3/2) for the LHY phase term, in FP64, from a density of precision T
-// (the square root in T: FP32 for the bulk path).
+// |c|^2 in FP64 from a value of precision T: exact products of FP32
+// components and one FP64 rounding, so a large nonlinear phase (tens of
+// radians per step) does not inherit the FP32 rounding of the density.
 template <typename T>
-__device__ __forceinline__ double rho32(T nrm) {
-    return (double)nrm * (double)sqrt(nrm);
+__device__ __forceinline__ double dens64(cplx<T> c) {
+    return (double)c.x * (double)c.x + (double)c.y * (double)c.y;
 }
 
 // Deterministic block sum of one double per thread (fixed tree).
@@ -992,9 +993,9 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                             // phase argument in FP64, reduced in FP64 before the
                             // sin / cos of the bulk precision, so a large phase (dense
                             // states, long steps) keeps its absolute accuracy
-                            const double dn = (double)nrm;
+                            const double dn = dens64(c);
                             const double vv = kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
-                            const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
+                            const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * (dn * sqrt(dn));
                             c = gpe::phase_rot(c, -a.hdt * veff);
                         }
                         Q[qi] = c;
@@ -1399,9 +1400,9 @@ __global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
                         }
                     }
                     if (!done) {
-                        const double dn = (double)nrm;
+                        const double dn = dens64(c);
                         const double veff =
-                            a.cv * ((double)a.V[e] + a.cdd * (double)phi_r[q]) + pg * dn + pgm * rho32(nrm);
+                            a.cv * ((double)a.V[e] + a.cdd * (double)phi_r[q]) + pg * dn + pgm * (dn * sqrt(dn));
                         c = gpe::phase_rot(c, -a.hdt * veff);
                     }
                     nrm = c.x * c.x + c.y * c.y;
@@ -2021,8 +2022,8 @@ __global__ void k_potential_half(cplx<T>* psi, const T* V, const T* phi, int nx,
         const int i = e / ((size_t)ny * nz);
         if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > nz - 2) continue;
         cplx<T> c = psi[e];
-        const T nr = c.x * c.x + c.y * c.y;
-        const double veff = (double)V[e] + cdd * (double)phi[e] + pg * (double)nr + pgm * rho32(nr);
+        const double dn = dens64(c);
+        const double veff = (double)V[e] + cdd * (double)phi[e] + pg * dn + pgm * (dn * sqrt(dn));
         psi[e] = gpe::phase_rot(c, -hdt * veff);
     }
 }

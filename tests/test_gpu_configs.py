@@ -23,11 +23,15 @@ pytestmark = pytest.mark.gpu
 SHAPE = (128, 128, 32)
 D = (120.0, 120.0, 600.0)
 SEED = 20261203
-# FP32 against FP64 after the 3000 real-time steps (measured on MI355X:
-# norm -2.6e-7, psi 1.8e-5, E/N 4.4e-7, mu 8.9e-7)
-NORM_DRIFT_FP32 = 2e-6
-PSI_RT_FP32 = 1e-4
-EN_RT_FP32 = 5e-6
+# FP32 against FP64 after the 3000 real-time steps, about twice what MI355X
+# measures (norm -2.6e-7, psi 1.8e-5, E/N 4.4e-7, mu 8.9e-7).  The psi error
+# is FP32 round-off of the transforms accumulating over the steps (its
+# amplitude part grows as sqrt(steps)); the dynamics do not amplify it: an
+# FP64 run started 8.5e-8 away from another stays 9e-8 away after 3000 steps.
+NORM_DRIFT_FP32 = 6e-7
+PSI_RT_FP32 = 4e-5
+EN_RT_FP32 = 1e-6
+MU_RT_FP32 = 2e-6
 
 
 def rel(a, b):
@@ -87,7 +91,7 @@ def test_full_simulation_fp32_vs_fp64_128x128x32():
     assert abs(n64 - 1.0) < 1e-12
     assert abs(n32 - 1.0) < NORM_DRIFT_FP32
     assert err_it["psi"] < 1e-5 and err_it["E/N"] < 1e-6 and err_it["mu"] < 1e-6
-    assert err_rt["psi"] < PSI_RT_FP32 and err_rt["E/N"] < EN_RT_FP32 and err_rt["mu"] < EN_RT_FP32
+    assert err_rt["psi"] < PSI_RT_FP32 and err_rt["E/N"] < EN_RT_FP32 and err_rt["mu"] < MU_RT_FP32
     for d in runs.values():
         d.close()
 
@@ -121,7 +125,7 @@ def test_first_steps_against_oracle_128x128x32(oracle):
         r = rel(dev.export_psi(), orc.psi)
         print(f"{'FP64' if prec == hip.GPE_FP64 else 'FP32'} +20 RT vs oracle: psi {r:.3e}  "
               f"E/N {abs(e['e_total'] - eo_rt['e_total']) / abs(eo_rt['e_total']):.3e}", flush=True)
-        assert r < (1e-10 if prec == hip.GPE_FP64 else 2e-5)
+        assert r < (1e-10 if prec == hip.GPE_FP64 else 2.5e-6)  # FP32 measured 1.0e-6
         assert abs(e["e_total"] - eo_rt["e_total"]) < tol * abs(eo_rt["e_total"])
         dev.close()
 

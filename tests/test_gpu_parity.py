@@ -3,7 +3,16 @@ on identical seeded inputs.
 
 Tolerances (relative L2 unless stated), from the solver's precision contract:
   FP64 path: psi <= 1e-10, Phi <= 1e-12, energies <= 1e-10 relative
-  FP32 path: psi <= 1e-5 (IT), <= 1e-6 per RT step, Phi <= 1e-5, E/N <= 1e-6 relative
+  FP32 path: psi <= 1e-5 (IT), Phi <= 1e-5, E/N <= 1e-6 relative; the FP32
+  real-time bounds are about twice the errors measured on MI355X (printed).
+
+FP32 real time at two densities.  At 500 atoms these grids hold ~0.05
+atoms per point, as the 128x128x32, 2e4-atom workload does: a step's
+nonlinear phase is below 0.1 rad and the FP32 error is the round-off of the
+transforms (~2e-7 per step).  At 2e4 atoms the nonlinear phase reaches tens
+of radians per step; the phase then carries the relative rounding of the
+stored FP32 psi (~6e-8 of |psi|^2 each) times that phase, and the error per
+step is ~1e-6.
 """
 import numpy as np
 import pytest
@@ -16,6 +25,18 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(16, 16, 8), (24, 12, 6), (32, 24, 12)]
 TOL_PSI = {hip.GPE_FP64: 1e-10, hip.GPE_FP32: 1e-5}
 TOL_PHI = {hip.GPE_FP64: 1e-12, hip.GPE_FP32: 1e-5}
+# FP32 real-time bounds at 2e4 atoms on the small grids, about twice what
+# MI355X measures: 1 step 1.8e-6 / 5.9e-7 / 2.6e-7; 20 steps 4.7e-5 /
+# 3.1e-4 / 7.2e-6 (the dense double wells amplify any perturbation: the
+# FP64 path grows 100-fold over the same 20 steps, 2e-14 -> 2e-12); Phi
+# <= 1.2e-6.  Single well, 20 steps: 3.4e-5 / 3.3e-5 / 1.4e-6; the edd ramp
+# (24x12x6, 12 steps): 2.4e-5.
+RT32_DENSE_1 = 4e-6
+RT32_DENSE_20 = {(16, 16, 8): 1e-4, (24, 12, 6): 6e-4, (32, 24, 12): 1.5e-5}
+RT32_DENSE_PHI = 2.5e-6
+SINGLE_WELL_TOL = {500.0: {s: 2e-6 for s in SHAPES},
+                   2e4: {(16, 16, 8): 7e-5, (24, 12, 6): 7e-5, (32, 24, 12): 3e-6}}
+RAMP_FP32_TOL = 5e-5
 
 
 def rel(a, b):
@@ -146,37 +167,56 @@ def test_potential_half_step(oracle, shape, prec):
     assert rel(dev.export_psi(), orc.psi) < TOL_PSI[prec]
 
 
-@pytest.mark.parametrize("shape", SHAPES)
-def test_rt_fused_steps(oracle, shape, prec):
-    # FP32 runs at 500 atoms: ~0.05 atoms per grid point as in the 128x128x32,
-    # 2e4-atom configuration.  (At 2e4 atoms on these small grids the
-    # nonlinear phase reaches tens of radians per step and the FP32 storage of
-    # psi alone limits a step to ~1e-5.)
-    p, dev, orc = make_pair(oracle, shape, prec, n_atoms=2e4 if prec == hip.GPE_FP64 else 500.0)
+# FP32 bounds of the fused real-time loop per density: (1 step, 20 steps,
+# Phi); at 500 atoms MI355X measures <= 1.7e-7, 8.4e-7 and 5.4e-7
+RT32_TOL = {500.0: lambda s: (4e-7, 2e-6, 1.2e-6),
+            2e4: lambda s: (RT32_DENSE_1, RT32_DENSE_20[s], RT32_DENSE_PHI)}
+
+
+def _rt_fused(oracle, shape, prec, n_atoms):
+    p, dev, orc = make_pair(oracle, shape, prec, n_atoms=n_atoms)
     # start RT from a Phi computed on the loaded state (FS hand-over)
     orc.calc_fi3d()
     dev.import_fi3d(orc.fi3d.copy())
     dev.rt_steps(1)
     orc.rt_steps(0, 1)
-    # one step: FP32 error is the FFT round-off floor (~1-3e-7)
-    assert rel(dev.export_psi(), orc.psi) < (TOL_PSI[prec] if prec == hip.GPE_FP64 else 1e-6)
+    e1 = rel(dev.export_psi(), orc.psi)
     dev.rt_steps(19)
     orc.rt_steps(1, 19)
-    # 20 steps: round-off accumulates (and double-well dynamics amplify it)
-    tol = TOL_PSI[prec] if prec == hip.GPE_FP64 else 2e-5
-    assert rel(dev.export_psi(), orc.psi) < tol
-    assert rel(dev.export_fi3d(), orc.fi3d) < (10 * TOL_PHI[prec] if prec == hip.GPE_FP64 else 2e-5)
+    e20 = rel(dev.export_psi(), orc.psi)
+    ef = rel(dev.export_fi3d(), orc.fi3d)
+    print(f"\nRT fused {shape} fp{prec} {n_atoms:g} atoms: 1 step {e1:.2e}, 20 steps {e20:.2e}, fi3d {ef:.2e}",
+          flush=True)
+    if prec == hip.GPE_FP64:
+        assert e1 < TOL_PSI[prec] and e20 < TOL_PSI[prec] and ef < 10 * TOL_PHI[prec]
+    else:
+        t1, t20, tf = RT32_TOL[n_atoms](shape)
+        assert e1 < t1 and e20 < t20 and ef < tf
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-def test_rt_fused_single_well_fp32(oracle, shape):
-    """Single well (HARMONIC, GAUSS): FP32 real time stays within 2e-5 over 20 steps."""
-    p, dev, orc = make_pair(oracle, shape, hip.GPE_FP32, pot="HARMONIC", init="GAUSS", n_atoms=500.0)
+def test_rt_fused_steps(oracle, shape, prec):
+    _rt_fused(oracle, shape, prec, 2e4 if prec == hip.GPE_FP64 else 500.0)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_rt_fused_steps_dense_fp32(oracle, shape):
+    """FP32 at 2e4 atoms on the small grids (tens of radians of phase per step)."""
+    _rt_fused(oracle, shape, hip.GPE_FP32, 2e4)
+
+
+@pytest.mark.parametrize("n_atoms", [500.0, 2e4])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_rt_fused_single_well_fp32(oracle, shape, n_atoms):
+    """Single well (HARMONIC, GAUSS), FP32, 20 real-time steps."""
+    p, dev, orc = make_pair(oracle, shape, hip.GPE_FP32, pot="HARMONIC", init="GAUSS", n_atoms=n_atoms)
     orc.calc_fi3d()
     dev.import_fi3d(orc.fi3d.copy())
     dev.rt_steps(20)
     orc.rt_steps(0, 20)
-    assert rel(dev.export_psi(), orc.psi) < 2e-5
+    e = rel(dev.export_psi(), orc.psi)
+    print(f"\nRT single well {shape} fp32 {n_atoms:g} atoms: 20 steps {e:.2e}", flush=True)
+    assert e < SINGLE_WELL_TOL[n_atoms][shape]
 
 
 def test_rt_from_zero_fi3d(oracle):
@@ -209,8 +249,9 @@ def test_rt_edd_ramp(oracle, prec):
     gm = np.array([p.coupling_at(t)[2] for t in range(n)])
     dev.rt_steps(n, g, gm)
     orc.rt_steps(0, n)
-    tol = TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-4  # see test_rt_fused_steps
-    assert rel(dev.export_psi(), orc.psi) < tol
+    e = rel(dev.export_psi(), orc.psi)
+    print(f"\nRT edd ramp fp{prec} 2e4 atoms, 12 steps: {e:.2e}", flush=True)
+    assert e < (TOL_PSI[prec] if prec == hip.GPE_FP64 else RAMP_FP32_TOL)
 
 
 @pytest.mark.parametrize("shape", SHAPES)

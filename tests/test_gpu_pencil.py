@@ -91,8 +91,12 @@ def test_pencil_pass_against_oracle_and_plane_pass(oracle, shape, tiles, monkeyp
         dev.rt_steps(4)
         ref.rt_steps(4)
         orc.rt_steps(0, 4)
-        assert rel(dev.export_psi(), orc.psi) < (tol if prec == hip.GPE_FP64 else 3e-6)
-        assert rel(dev.export_psi(), ref.export_psi()) < (1e-13 if prec == hip.GPE_FP64 else 2e-6)
+        e_o, e_r = rel(dev.export_psi(), orc.psi), rel(dev.export_psi(), ref.export_psi())
+        print(f"\npencil {shape} tiles {tiles} fp{prec}: 4 RT vs oracle {e_o:.2e}, vs plane pass {e_r:.2e}",
+              flush=True)
+        # FP32: MI355X measures <= 3.6e-7 for both
+        assert e_o < (tol if prec == hip.GPE_FP64 else 8e-7)
+        assert e_r < (1e-13 if prec == hip.GPE_FP64 else 8e-7)
         assert rel(dev.export_fi3d(), orc.fi3d) < (1e-11 if prec == hip.GPE_FP64 else 1e-5)
         dev.close()
         ref.close()
@@ -137,7 +141,7 @@ def test_large_rt_first_steps_against_oracle(oracle, monkeypatch):
         r = rel(dev.export_psi(), orc.psi)
         print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} (GPE_PENCIL_YREG={yreg}) 3 RT steps vs "
               f"oracle: psi {r:.3e}", flush=True)
-        assert r < (1e-10 if prec == hip.GPE_FP64 else 2e-6)
+        assert r < (1e-10 if prec == hip.GPE_FP64 else 8e-7)  # FP32: 3.4e-7 measured
         dev.close()
 
 
@@ -168,9 +172,10 @@ def test_large_it_and_poisson_against_oracle(oracle, monkeypatch):
         print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} (GPE_PENCIL_YREG={yreg}) vs oracle: "
               f"fi3d {rf:.3e}, psi after 3 IT steps {r:.3e}, E {de:.3e}", flush=True)
         fp64 = prec == hip.GPE_FP64
-        assert rf < (1e-11 if fp64 else 1e-5)
-        assert r < (1e-10 if fp64 else 2e-6)
-        assert de < (1e-10 if fp64 else 1e-6)
+        # FP32: fi3d 1.45e-7, psi 5.9e-8, E 5.6e-8 measured
+        assert rf < (1e-11 if fp64 else 3e-7)
+        assert r < (1e-10 if fp64 else 1.5e-7)
+        assert de < (1e-10 if fp64 else 1.2e-7)
         dev.close()
     assert rel(out[(hip.GPE_FP64, "1")], out[(hip.GPE_FP64, "0")]) < 1e-13
     assert rel(out[(hip.GPE_FP32, "1")], out[(hip.GPE_FP32, "0")]) < 1e-6
@@ -199,8 +204,9 @@ def test_large_rt_500_steps_fp32_vs_fp64():
     psi_err = rel(devs[hip.GPE_FP32].export_psi(), devs[hip.GPE_FP64].export_psi())
     print(f"after 500 steps: FP32 psi vs FP64 {psi_err:.2e}, max norm drift {worst_norm:.2e}, "
           f"max E/N error {worst_e:.2e} (E/N at t=0: {e0[hip.GPE_FP64] / p.n_atoms:.9e})", flush=True)
-    assert worst_norm < 1e-6
-    assert worst_e < 1e-5
-    assert psi_err < 1e-4
+    # about twice what MI355X measures (norm 6.1e-8, E/N 6.1e-8, psi 4.6e-6)
+    assert worst_norm < 1.5e-7
+    assert worst_e < 1.5e-7
+    assert psi_err < 1e-5
     for d in devs.values():
         d.close()
