@@ -67,18 +67,36 @@ __global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, 
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 // column sums of a [rows x cols] row-major matrix (bias gradients), fixed order
-__global__ void k_colsum(const float* __restrict__ x, long rows, int cols, float* __restrict__ out) {
-    const int c = blockIdx.x;
+// Column sums, stage 1: block b adds rows [b rpb, (b+1) rpb) of x[rows][cols]
+// (cols divides 256): thread t takes column t % cols and every (256/cols)-th
+// row, so a wave reads whole rows; part[b][c] = the block's sum, added over
+// the row groups in a fixed order.
+__global__ void __launch_bounds__(256) k_colsum_part(const float* __restrict__ x, long rows, int cols, long rpb,
+                                                     float* __restrict__ part) {
     __shared__ float red[256];
+    const int c = threadIdx.x % cols, g = threadIdx.x / cols, ng = 256 / cols;
+    const long r0 = (long)blockIdx.x * rpb, r1 = min(rows, r0 + rpb);
     float acc = 0.f;
-    for (long r = threadIdx.x; r < rows; r += blockDim.x) acc += x[r * cols + c];
+    for (long r = r0 + g; r < r1; r += ng) acc += x[r * cols + c];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x < cols) {
+        float s = 0.f;
+        for (int q = 0; q < ng; ++q) s += red[q * cols + threadIdx.x];
+        part[(long)blockIdx.x * cols + threadIdx.x] = s;
     }
-    if (threadIdx.x == 0) out[c] = red[0];
+}
+// Split-K reduction, stage 1 of two: block (x, s) adds chunks
+// [s cps, (s+1) cps) of part[chunk][n] for 256 consecutive elements;
+// out2[s][e] = that sum (chunks in order).
+__global__ void __launch_bounds__(256) k_sum_chunks_part(const float* __restrict__ part, int nchunk, int cps, long n,
+                                                         float* __restrict__ out2) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int c0 = blockIdx.y * cps, c1 = min(nchunk, c0 + cps);
+    float acc = 0.f;
+    for (int c = c0; c < c1; ++c) acc += part[(long)c * n + e];
+    out2[(long)blockIdx.y * n + e] = acc;
 }
 // out[i] = sum_c part[c][i]  (split-K reduction)
 __global__ void k_sum_chunks(const float* __restrict__ part, int nchunk, long n, float* __restrict__ out) {
@@ -111,6 +129,8 @@ struct Tensor {
 //   W_lift [cin][W], b_lift [W]
 //   per layer: R_re [nm][W][W], R_im [nm][W][W], W_l [W][W], b_l [W]
 //   W_q1 [W][H], b_q1 [H], W_q2 [H], b_q2 [1]
+constexpr size_t kKpartFloats = (size_t)64 * 1024 * 1024;  // split-K / reduction scratch
+
 struct gpe_fno {
     int nx, ny, nz, cin, width, layers, hidden, mx, my, mz, maxb;
     int nm;      // retained modes
@@ -152,6 +172,13 @@ int ffail(gpe_fno* f, const std::string& m) {
 
 int gemm(gpe_fno* f, const Args& a) {
     if (a.M <= 0 || a.N <= 0 || a.K <= 0 || a.batch <= 0) return 0;
+    if (gpe::gemm::small_ok(a) && a.batch >= 256) {
+        const long per = 256 / a.N;
+        hipLaunchKernelGGL(gpe::gemm::k_gemm_small, dim3((unsigned)((a.batch + per - 1) / per)), dim3(256), 0,
+                           f->stream, a);
+        FCHECK(hipGetLastError());
+        return 0;
+    }
     dim3 grid((a.N + gpe::gemm::TN - 1) / gpe::gemm::TN, (a.M + gpe::gemm::TM - 1) / gpe::gemm::TM, a.batch);
     if (grid.z > 65535) return ffail(f, "gemm batch too large");
     hipLaunchKernelGGL(gpe::gemm::k_gemm, grid, dim3(gpe::gemm::NT), 0, f->stream, a);
@@ -174,8 +201,8 @@ Args mkargs(int M, int N, int K, int batch) {
 // is handled by callers): C[M x N] = A[M x K] . B[K x N] (+bias) (+beta C)
 int gemm_rm(gpe_fno* f, const float* A, const float* B, float* C, long M, int N, int K, bool transA, bool transB,
             const float* bias, float beta) {
-    // split M into batches of <= 2^20 rows so grid.y stays small
-    const long chunk = 1L << 16;
+    // one launch per 2^21 rows (grid.y = 32768 tiles of 64 rows at most)
+    const long chunk = 1L << 21;
     for (long m0 = 0; m0 < M; m0 += chunk * 1) {
         const int mm = (int)std::min(chunk, M - m0);
         Args a = mkargs(mm, N, K, 1);
@@ -207,9 +234,25 @@ int gemm_rm(gpe_fno* f, const float* A, const float* B, float* C, long M, int N,
     return 0;
 }
 
+// out[e] = sum_c part[c][e] for nchunk chunks of n elements, in a fixed
+// order: groups of 32 chunks in parallel, then the groups in order.  The
+// group sums go to kpart after the nchunk * n partials.
+int sum_chunks(gpe_fno* f, const float* part, int nchunk, long n, float* out) {
+    const int cps = 32, ns = (nchunk + cps - 1) / cps;
+    float* part2 = f->kpart + (size_t)nchunk * n;
+    if ((size_t)(nchunk + ns) * n > kKpartFloats) return ffail(f, "split-K scratch too small");
+    hipLaunchKernelGGL(k_sum_chunks_part, dim3((unsigned)((n + 255) / 256), ns), dim3(256), 0, f->stream, part, nchunk,
+                       cps, n, part2);
+    FCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_chunks, dim3(256), dim3(256), 0, f->stream, part2, ns, n, out);
+    FCHECK(hipGetLastError());
+    return 0;
+}
+
 // dW[K x N] = A^T[K x M] . D[M x N] over M = rows (large), split-K in chunks.
 int gemm_wgrad(gpe_fno* f, const float* A, const float* D, float* dW, long M, int K, int N) {
-    const long chunk = 8192;
+    // 512-row chunks: M = 2^19 rows give 1024 blocks (one per chunk per tile)
+    const long chunk = 512;
     const int nch = (int)((M + chunk - 1) / chunk);
     Args a = mkargs(K, N, (int)chunk, nch);
     a.Ar = A;
@@ -224,12 +267,21 @@ int gemm_wgrad(gpe_fno* f, const float* A, const float* D, float* dW, long M, in
     a.scm = N;
     a.scn = 1;
     a.scz = (long)K * N;
-    if (M % chunk != 0) return ffail(f, "rows must be a multiple of 8192");
-    if ((size_t)nch * K * N > (size_t)64 * 1024 * 1024) return ffail(f, "split-K scratch too small");
+    if (M % chunk != 0) return ffail(f, "rows must be a multiple of 512");
+    if ((size_t)nch * K * N > kKpartFloats / 2) return ffail(f, "split-K scratch too small");
     if (gemm(f, a)) return 1;
-    hipLaunchKernelGGL(k_sum_chunks, dim3(256), dim3(256), 0, f->stream, f->kpart, nch, (long)K * N, dW);
+    return sum_chunks(f, f->kpart, nch, (long)K * N, dW);
+}
+
+// out[c] = sum_r x[r][c] (cols divides 256), deterministic: per-block
+// partials in kpart, then their fixed-order sum.
+int colsum(gpe_fno* f, const float* x, long rows, int cols, float* out) {
+    if (cols <= 0 || 256 % cols != 0) return ffail(f, "colsum: columns must divide 256");
+    const int nb = 512;
+    const long rpb = (rows + nb - 1) / nb;
+    hipLaunchKernelGGL(k_colsum_part, dim3(nb), dim3(256), 0, f->stream, x, rows, cols, rpb, f->kpart);
     FCHECK(hipGetLastError());
-    return 0;
+    return sum_chunks(f, f->kpart, nb, (long)cols, out);
 }
 
 void fill_dft(std::vector<float>& re, std::vector<float>& im, int rows, int cols, const std::vector<int>& kl, int n,
@@ -563,12 +615,12 @@ int fno_backward(gpe_fno* f, int B) {
     FCHECK(hipMemsetAsync(f->g, 0, f->nparam * 4, f->stream));
     // y = q Wq2 + bq2
     if (gemm_wgrad(f, f->q, f->dy, f->g + f->off_q2_w, R, H, 1)) return 1;
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, f->stream, f->dy, R, 1, f->g + f->off_q2_b);
+    if (colsum(f, f->dy, R, 1, f->g + f->off_q2_b)) return 1;
     if (gemm_rm(f, f->dy, f->w + f->off_q2_w, f->dq, R, H, 1, false, true, nullptr, 0.f)) return 1;
     hipLaunchKernelGGL(k_gelu_bwd, dim3(2048), dim3(256), 0, f->stream, f->dq, f->q_pre, f->dq, (size_t)(R * H));
     // q_pre = h_L Wq1 + bq1
     if (gemm_wgrad(f, f->h[f->layers], f->dq, f->g + f->off_q1_w, R, W, H)) return 1;
-    hipLaunchKernelGGL(k_colsum, dim3(H), dim3(256), 0, f->stream, f->dq, R, H, f->g + f->off_q1_b);
+    if (colsum(f, f->dq, R, H, f->g + f->off_q1_b)) return 1;
     if (gemm_rm(f, f->dq, f->w + f->off_q1_w, f->dh, R, W, H, false, true, nullptr, 0.f)) return 1;
     for (int l = f->layers - 1; l >= 0; --l) {
         if (l < f->layers - 1)
@@ -577,12 +629,12 @@ int fno_backward(gpe_fno* f, int B) {
         else
             FCHECK(hipMemcpyAsync(f->dz, f->dh, (size_t)R * W * 4, hipMemcpyDeviceToDevice, f->stream));
         if (gemm_wgrad(f, f->h[l], f->dz, f->g + f->off_w[l], R, W, W)) return 1;
-        hipLaunchKernelGGL(k_colsum, dim3(W), dim3(256), 0, f->stream, f->dz, R, W, f->g + f->off_b[l]);
+        if (colsum(f, f->dz, R, W, f->g + f->off_b[l])) return 1;
         if (gemm_rm(f, f->dz, f->w + f->off_w[l], f->dh, R, W, W, false, true, nullptr, 0.f)) return 1;
         if (spectral_backward(f, l, B, f->dz, f->dh)) return 1;
     }
     if (gemm_wgrad(f, f->xin, f->dh, f->g + f->off_lift_w, R, f->cin, W)) return 1;
-    hipLaunchKernelGGL(k_colsum, dim3(W), dim3(256), 0, f->stream, f->dh, R, W, f->g + f->off_lift_b);
+    if (colsum(f, f->dh, R, W, f->g + f->off_lift_b)) return 1;
     FCHECK(hipGetLastError());
     return 0;
 }
@@ -704,7 +756,7 @@ int gpe_fno_create(const gpe_fno_config* c, unsigned long long seed, gpe_fno** o
          A(&f->c2i, std::max(cmid, cbig)) && A(&f->q_pre, (size_t)R * H) && A(&f->q, (size_t)R * H) &&
          A(&f->y, R) && A(&f->t, R) && A(&f->dh, (size_t)R * W) && A(&f->dz, (size_t)R * W) &&
          A(&f->dq, (size_t)R * H) && A(&f->dy, R) && A(&f->dtmp, 2 * (size_t)f->maxb * f->nm * W) &&
-         A(&f->kpart, (size_t)64 * 1024 * 1024) && hipMalloc(&f->lpart, 256 * 8) == hipSuccess;
+         A(&f->kpart, kKpartFloats) && hipMalloc(&f->lpart, 256 * 8) == hipSuccess;
     for (int t = 0; t < 2 && ok; ++t)
         ok = A(&f->fz[t], (size_t)f->mz * f->nz) && A(&f->fy[t], (size_t)f->my * f->ny) &&
              A(&f->fx[t], (size_t)f->mx * f->nx) && A(&f->gx[t], (size_t)f->nx * f->mx) &&

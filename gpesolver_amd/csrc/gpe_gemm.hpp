@@ -11,7 +11,8 @@
 // only Re(C).  conjA / conjB conjugate the operand.
 //
 // Tile: 64x64 per block of 256 threads (4 waves, 32x32 each = 2x2 MFMA
-// 16x16x32 tiles), K staged 32 at a time.
+// 16x16x32 tiles), K staged 32 at a time.  k_gemm_small (below) takes the
+// batches of tiny products with a shared A in FP32.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -177,6 +178,69 @@ __global__ void __launch_bounds__(NT) k_gemm(Args a) {
                     }
                 }
             }
+}
+
+// Many small products with one shared A (saz == 0): M <= 16, K <= 32 and N
+// dividing 256 (the z DFTs of the spectral layers, 9 x 16 by 16 x 32 per
+// grid line).  They are memory-bound, so FP32 FMA on one thread per output
+// column n of one product beats 64 x 64 MFMA tiles that such shapes fill to a
+// few percent: a block runs 256 / N products, B and C are read and written
+// along n (contiguous when scn = sbn = 1), and A sits in LDS.
+constexpr int SM_M = 16, SM_K = 32;
+
+inline bool small_ok(const Args& a) {
+    return a.saz == 0 && a.M <= SM_M && a.K <= SM_K && a.N > 0 && 256 % a.N == 0 && a.bias == nullptr;
+}
+
+__global__ void __launch_bounds__(256) k_gemm_small(Args a) {
+    __shared__ float far[SM_M * SM_K], fai[SM_M * SM_K];
+    const bool cA = a.Ai != nullptr, cB = a.Bi != nullptr, cplx_out = a.Ci != nullptr;
+    for (int e = threadIdx.x; e < a.M * a.K; e += 256) {
+        const int m = e / a.K, k = e - m * a.K;
+        const long off = (long)m * a.sam + (long)k * a.sak;
+        far[e] = a.Ar[off];
+        fai[e] = cA ? (a.conjA ? -a.Ai[off] : a.Ai[off]) : 0.f;
+    }
+    __syncthreads();
+    const int per = 256 / a.N;
+    const int n = threadIdx.x % a.N;
+    const long z = (long)blockIdx.x * per + threadIdx.x / a.N;
+    if (z >= a.batch) return;
+    const float sgnB = a.conjB ? -1.f : 1.f;
+    const float* Br = a.Br + z * a.sbz + (long)n * a.sbn;
+    const float* Bi = cB ? a.Bi + z * a.sbz + (long)n * a.sbn : nullptr;
+    float br[SM_K], bi[SM_K];
+#pragma unroll
+    for (int k = 0; k < SM_K; ++k) {
+        br[k] = 0.f;
+        bi[k] = 0.f;
+        if (k < a.K) {
+            br[k] = Br[(long)k * a.sbk];
+            if (cB) bi[k] = sgnB * Bi[(long)k * a.sbk];
+        }
+    }
+    float* Cr = a.Cr + z * a.scz + (long)n * a.scn;
+    float* Ci = cplx_out ? a.Ci + z * a.scz + (long)n * a.scn : nullptr;
+    for (int m = 0; m < a.M; ++m) {
+        float sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int k = 0; k < SM_K; ++k) {
+            if (k < a.K) {
+                const float ar = far[m * a.K + k], ai = fai[m * a.K + k];
+                sr += ar * br[k] - ai * bi[k];
+                si += ar * bi[k] + ai * br[k];
+            }
+        }
+        const long off = (long)m * a.scm;
+        float vr = a.alpha * sr;
+        if (a.beta != 0.f) vr += a.beta * Cr[off];
+        Cr[off] = vr;
+        if (cplx_out) {
+            float vi = a.alpha * si;
+            if (a.beta != 0.f) vi += a.beta * Ci[off];
+            Ci[off] = vi;
+        }
+    }
 }
 
 }  // namespace gemm
