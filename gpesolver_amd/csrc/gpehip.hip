@@ -2033,13 +2033,9 @@ __global__ void k_potential_half(cplx<T>* psi, const T* V, const T* phi, int nx,
 //          sum |phi|^4, sum Phi|phi|^2, sum |phi|^5, sum Re(conj phi H phi)}
 // pass 1: out[i*8 + 6] = sum |H phi - mu phi|^2
 template <typename T, int NT>
-__global__ void __launch_bounds__(NT) k_energy(const cplx<T>* psi, const T* V, const T* phi, int nx, int ny, int nz,
-                                               double ix2, double iy2, double iz2, double m, double cdd, double gq,
-                                               double gmq, double mu, int pass, double* out,
-                                               const double* mu_dev = nullptr) {
-    __shared__ double red[NT / 64 + 1];
-    const int i = blockIdx.x;
-    if (mu_dev) mu = *mu_dev;
+__device__ __forceinline__ void energy_plane(const cplx<T>* psi, const T* V, const T* phi, int i, int nx, int ny,
+                                             int nz, double ix2, double iy2, double iz2, double m, double cdd,
+                                             double gq, double gmq, double mu, int pass, double* out, double* red) {
     const int npl = ny * nz;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (i >= 1 && i <= nx - 2) {
@@ -2092,14 +2088,74 @@ __global__ void __launch_bounds__(NT) k_energy(const cplx<T>* psi, const T* V, c
     }
 }
 
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_energy(const cplx<T>* psi, const T* V, const T* phi, int nx, int ny, int nz,
+                                               double ix2, double iy2, double iz2, double m, double cdd, double gq,
+                                               double gmq, double mu, int pass, double* out,
+                                               const double* mu_dev = nullptr) {
+    __shared__ double red[NT / 64 + 1];
+    if (mu_dev) mu = *mu_dev;
+    energy_plane<T, NT>(psi, V, phi, blockIdx.x, nx, ny, nz, ix2, iy2, iz2, m, cdd, gq, gmq, mu, pass, out, red);
+}
+
+// Per-instance arguments of the batched energy check: the state (scaled in
+// place by 1/sqrt(sum of its norm partials), as k_scale does), the k_energy
+// inputs, the partials (the instance's epart), and sums / out (8 doubles
+// each) of k_energy_sums.
+template <typename T>
+struct EArgs {
+    cplx<T>* psi[2];  // the two state buffers; the batch's parity picks one
+    const T* V;
+    const T* phi;
+    const double* npart[2];  // per-plane sum |phi|^2 of psi, by parity
+    double* part;
+    double* sums;
+    double* out;
+    size_t n;
+    int nx, ny, nz;
+    double ix2, iy2, iz2, m, cdd, gq, gmq;
+    double N, dV, g, gamma, dv15, n25;  // dv15 = dV^1.5, n25 = N^2.5 (host pow)
+};
+
+constexpr int kScaleChunk = 16384;  // elements per block of k_scale_group
+
+// k_scale over the active instances of a batch: work (instance, chunk).
+template <typename T>
+__global__ void __launch_bounds__(256) k_scale_group(const EArgs<T>* __restrict__ args, const int2* __restrict__ work,
+                                                     int cur, int pcur) {
+    __shared__ double sred[2];
+    const int2 w = work[blockIdx.x];
+    const EArgs<T>& a = args[w.x];
+    const T sc = (T)(1.0 / sqrt(sum_partials(a.npart[pcur], a.nx, sred)));
+    cplx<T>* psi = a.psi[cur];
+    const size_t e1 = min(a.n, (size_t)(w.y + 1) * kScaleChunk);
+    for (size_t e = (size_t)w.y * kScaleChunk + threadIdx.x; e < e1; e += 256) {
+        cplx<T> v = psi[e];
+        psi[e] = gpe::mk(v.x * sc, v.y * sc);
+    }
+}
+
+// k_energy over the active instances: work (instance, plane); pass 1 takes
+// mu from the instance's sums[5].
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_energy_group(const EArgs<T>* __restrict__ args, const int2* __restrict__ work,
+                                                     int pass, int cur) {
+    __shared__ double red[NT / 64 + 1];
+    const int2 w = work[blockIdx.x];
+    const EArgs<T>& a = args[w.x];
+    const double mu = pass ? a.sums[5] : 0.0;
+    energy_plane<T, NT>(a.psi[cur], a.V, a.phi, w.y, a.nx, a.ny, a.nz, a.ix2, a.iy2, a.iz2, a.m, a.cdd, a.gq, a.gmq, mu,
+                        pass, a.part, red);
+}
+
 // Energies from the per-plane partials of k_energy, on the device: pass 0
 // adds the six pass-0 sums in plane order into sums[0..5] (mu = sums[5]);
 // pass 1 adds the residual sums and writes the eight outputs (gpe_energies
 // order) with the formulas and operation order of do_energy.
-__global__ void k_energy_sums(const double* part, int nx, int pass, double* sums, double* out, double m, double N,
-                              double dV, double g, double gamma, double cdd) {
+__device__ __forceinline__ void energy_sums(const double* part, int nx, int pass, double* sums, double* out, double m,
+                                            double N, double dV, double g, double gamma, double cdd, double dv15,
+                                            double n25) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (pass == 0) {
         double s[6] = {0, 0, 0, 0, 0, 0};
         for (int i = 0; i < nx; ++i)
@@ -2114,7 +2170,7 @@ __global__ void k_energy_sums(const double* part, int nx, int pass, double* sums
     const double e_pot = s[1] * N;
     const double e_int = 0.5 * g * s[2] / dV * N * N;
     const double e_ext = 0.5 * cdd * s[3] * N - cdd / 3. * s[2] / dV / 2. * N * N;
-    const double e_bmf = 2. / 5. * gamma * s[4] / pow(dV, 1.5) * pow(N, 2.5);
+    const double e_bmf = 2. / 5. * gamma * s[4] / dv15 * n25;
     out[0] = e_kin;
     out[1] = e_pot;
     out[2] = e_int;
@@ -2123,6 +2179,20 @@ __global__ void k_energy_sums(const double* part, int nx, int pass, double* sums
     out[5] = e_kin + e_pot + e_int + e_ext + e_bmf;
     out[6] = s[5];
     out[7] = sqrt(r2);
+}
+
+__global__ void k_energy_sums(const double* part, int nx, int pass, double* sums, double* out, double m, double N,
+                              double dV, double g, double gamma, double cdd, double dv15, double n25) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    energy_sums(part, nx, pass, sums, out, m, N, dV, g, gamma, cdd, dv15, n25);
+}
+
+// k_energy_sums of every active instance: block q, one thread.
+template <typename T>
+__global__ void k_energy_sums_group(const EArgs<T>* __restrict__ args, int pass) {
+    if (threadIdx.x != 0) return;
+    const EArgs<T>& a = args[blockIdx.x];
+    energy_sums(a.part, a.nx, pass, a.sums, a.out, a.m, a.N, a.dV, a.g, a.gamma, a.cdd, a.dv15, a.n25);
 }
 
 // Scratch of gpe_calc_energy_noalloc: per-plane partials (8 doubles per
@@ -3189,7 +3259,7 @@ static int do_energy_noalloc(gpe_handle* h, double* out_dev) {
                            pass ? sums + 5 : nullptr);
         GPE_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_energy_sums, dim3(1), dim3(64), 0, h->stream, part, p.nx, pass, sums, out_dev, p.m,
-                           p.n_atoms, dV, h->ggp11, h->gamma, p.cdd);
+                           p.n_atoms, dV, h->ggp11, h->gamma, p.cdd, pow(dV, 1.5), pow(p.n_atoms, 2.5));
         GPE_CHECK(hipGetLastError());
     }
     return 0;
@@ -3221,6 +3291,20 @@ struct gpe_batch {
     std::vector<Group> pgroups, lgroups;
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     bool use_graphs = true;
+    // Device buffers sized at the first setup for every instance and kept
+    // while the active set shrinks: argument blocks and work lists of the
+    // steps and of the energy check.
+    size_t cap_pargs = 0, cap_largs = 0, cap_pwork = 0, cap_lwork = 0, cap_eargs = 0, cap_swork = 0, cap_ework = 0;
+    void* d_eargs = nullptr;   // EArgs per active instance
+    void* d_swork = nullptr;   // (instance, chunk) of the grouped scale
+    void* d_ework = nullptr;   // (instance, plane) of the grouped energy
+    int nswork = 0, nework = 0;
+    double* d_esum = nullptr;  // per instance: sums[8], out[8]
+    double* h_eout = nullptr;  // pinned: out[8] per active instance
+    // Fallback states (gpe_batch_set_fallback): the state an instance
+    // restarts from when its own is non-finite or its residual grows.
+    std::vector<std::vector<double>> fallback;
+    std::vector<char> fell_back;
 };
 
 static int bfail(gpe_batch* b, const std::string& m) {
@@ -3320,22 +3404,102 @@ static int batch_setup(gpe_batch* b) {
         g.cnt = (int)lw.size() - g.off;
         b->lgroups.push_back(g);
     }
-    void* old[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork};
-    for (void* o : old)
-        if (o) BCHECK(hipFree(o));
-    b->d_pargs = b->d_largs = nullptr;
-    b->d_pwork = b->d_lwork = nullptr;
+    // energy check: per-instance arguments, (instance, chunk) and (instance,
+    // plane) work lists
+    std::vector<EArgs<T>> ea(na);
+    std::vector<int2> sw, ew;
+    for (size_t q = 0; q < na; ++q) {
+        gpe_handle* h = b->inst[b->active[q]];
+        const gpe_params& p = h->p;
+        const double dV = p.dx * p.dy * p.dz;
+        EArgs<T> e;
+        e.psi[0] = (cplx<T>*)h->store[0];
+        e.psi[1] = (cplx<T>*)h->store[1];
+        e.V = (const T*)h->V;
+        e.phi = (const T*)h->phi;
+        e.npart[0] = h->part[0];
+        e.npart[1] = h->part[1];
+        e.part = h->epart;
+        e.sums = b->d_esum + (size_t)b->active[q] * 16;
+        e.out = e.sums + 8;
+        e.n = h->N;
+        e.nx = p.nx;
+        e.ny = p.ny;
+        e.nz = p.nz;
+        e.ix2 = 1.0 / (p.dx * p.dx);
+        e.iy2 = 1.0 / (p.dy * p.dy);
+        e.iz2 = 1.0 / (p.dz * p.dz);
+        e.m = p.m;
+        e.cdd = p.cdd;
+        e.gq = (h->ggp11 - p.cdd / 3) * p.n_atoms / dV;
+        e.gmq = h->gamma * p.w_15 / pow(dV, 1.5);
+        e.N = p.n_atoms;
+        e.dV = dV;
+        e.g = h->ggp11;
+        e.gamma = h->gamma;
+        e.dv15 = pow(dV, 1.5);
+        e.n25 = pow(p.n_atoms, 2.5);
+        ea[q] = e;
+        for (size_t c = 0; c * kScaleChunk < h->N; ++c) sw.push_back(make_int2((int)q, (int)c));
+        for (int i = 0; i < p.nx; ++i) ew.push_back(make_int2((int)q, i));
+    }
+    // device buffers: sized by the first (largest) active set, then reused
+    auto ensure = [&](void** d, size_t& cap, size_t bytes) -> int {
+        if (bytes <= cap && *d) return 0;
+        if (*d) BCHECK(hipFree(*d));
+        *d = nullptr;
+        cap = 0;
+        BCHECK(hipMalloc(d, std::max<size_t>(bytes, 1)));
+        cap = bytes;
+        return 0;
+    };
     b->npwork = (int)pw.size();
     b->nlwork = (int)lw.size();
+    b->nswork = (int)sw.size();
+    b->nework = (int)ew.size();
     if (na == 0) return 0;
-    BCHECK(hipMalloc(&b->d_pargs, pa.size() * sizeof(PlaneArgs<T>)));
-    BCHECK(hipMalloc(&b->d_largs, la.size() * sizeof(LineArgs<T>)));
-    BCHECK(hipMalloc(&b->d_pwork, pw.size() * sizeof(int2)));
-    BCHECK(hipMalloc(&b->d_lwork, lw.size() * sizeof(int2)));
+    if (ensure(&b->d_pargs, b->cap_pargs, pa.size() * sizeof(PlaneArgs<T>)) ||
+        ensure(&b->d_largs, b->cap_largs, la.size() * sizeof(LineArgs<T>)) ||
+        ensure(&b->d_pwork, b->cap_pwork, pw.size() * sizeof(int2)) ||
+        ensure(&b->d_lwork, b->cap_lwork, lw.size() * sizeof(int2)) ||
+        ensure(&b->d_eargs, b->cap_eargs, ea.size() * sizeof(EArgs<T>)) ||
+        ensure(&b->d_swork, b->cap_swork, sw.size() * sizeof(int2)) ||
+        ensure(&b->d_ework, b->cap_ework, ew.size() * sizeof(int2)))
+        return 1;
     BCHECK(hipMemcpy(b->d_pargs, pa.data(), pa.size() * sizeof(PlaneArgs<T>), hipMemcpyHostToDevice));
     BCHECK(hipMemcpy(b->d_largs, la.data(), la.size() * sizeof(LineArgs<T>), hipMemcpyHostToDevice));
     BCHECK(hipMemcpy(b->d_pwork, pw.data(), pw.size() * sizeof(int2), hipMemcpyHostToDevice));
     BCHECK(hipMemcpy(b->d_lwork, lw.data(), lw.size() * sizeof(int2), hipMemcpyHostToDevice));
+    BCHECK(hipMemcpy(b->d_eargs, ea.data(), ea.size() * sizeof(EArgs<T>), hipMemcpyHostToDevice));
+    BCHECK(hipMemcpy(b->d_swork, sw.data(), sw.size() * sizeof(int2), hipMemcpyHostToDevice));
+    BCHECK(hipMemcpy(b->d_ework, ew.data(), ew.size() * sizeof(int2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Energy check of every active instance (what do_energy does per handle:
+// normalise, then the two k_energy passes and their sums), as five launches
+// over all of them and one copy of the results: out_host[q * 8 + k] for
+// active slot q, gpe_energies order.
+template <typename T>
+static int batch_check(gpe_batch* b, double* out_host) {
+    const int na = (int)b->active.size();
+    if (na == 0) return 0;
+    const EArgs<T>* ea = (const EArgs<T>*)b->d_eargs;
+    hipLaunchKernelGGL((k_scale_group<T>), dim3(b->nswork), dim3(256), 0, b->stream, ea, (const int2*)b->d_swork,
+                       b->cur, b->pcur);
+    BCHECK(hipGetLastError());
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL((k_energy_group<T, 256>), dim3(b->nework), dim3(256), 0, b->stream, ea,
+                           (const int2*)b->d_ework, pass, b->cur);
+        BCHECK(hipGetLastError());
+        hipLaunchKernelGGL((k_energy_sums_group<T>), dim3(na), dim3(64), 0, b->stream, ea, pass);
+        BCHECK(hipGetLastError());
+    }
+    for (int q = 0; q < na; ++q)
+        BCHECK(hipMemcpyAsync(out_host + (size_t)q * 8, b->d_esum + (size_t)b->active[q] * 16 + 8, 8 * sizeof(double),
+                              hipMemcpyDeviceToHost, b->stream));
+    BCHECK(hipStreamSynchronize(b->stream));
+    for (int q : b->active) b->inst[q]->norm_pending = false;
     return 0;
 }
 
@@ -3450,12 +3614,18 @@ static int batch_it(gpe_batch* b, long n) {
     return 0;
 }
 
+// Residual growth that sends a warm-started instance to its fallback state:
+// the residual ||H psi - mu psi|| exceeds kFallbackGrowth times the smallest
+// value it has had since the instance started.
+constexpr double kFallbackGrowth = 2.0;
+
 template <typename T>
 static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double tol, long* steps_out,
                           double* e_out) {
     const int ni = (int)b->inst.size();
-    std::vector<double> last(ni, 0.0);
+    std::vector<double> last(ni, 0.0), rmin(ni, 0.0);
     std::vector<char> have(ni, 0);
+    b->fell_back.assign(ni, 0);
     b->active.clear();
     for (int q = 0; q < ni; ++q) {
         b->active.push_back(q);
@@ -3463,21 +3633,36 @@ static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double
     }
     long steps = 0;
     if (check_every <= 0) check_every = 100;
+    if (!b->d_esum) BCHECK(hipMalloc((void**)&b->d_esum, (size_t)ni * 16 * sizeof(double)));
+    if (!b->h_eout) BCHECK(hipHostMalloc((void**)&b->h_eout, (size_t)ni * 8 * sizeof(double), hipHostMallocDefault));
     if (batch_setup<T>(b)) return 1;
     while (!b->active.empty() && steps < max_steps) {
         const long n = std::min(check_every, max_steps - steps);
         if (batch_it<T>(b, n)) return 1;
         steps += n;
+        if (batch_check<T>(b, b->h_eout)) return 1;
         std::vector<int> keep;
-        for (int q : b->active) {
+        for (size_t s = 0; s < b->active.size(); ++s) {
+            const int q = b->active[s];
             gpe_handle* h = b->inst[q];
-            gpe_energies e;
-            if (do_energy<T>(h, &e)) return bfail(b, h->err);
-            const double epa = e.e_total / h->p.n_atoms;
-            if (e_out) e_out[q] = epa;
+            const double* e = b->h_eout + s * 8;
+            const double epa = e[5] / h->p.n_atoms, res = e[7];
             if (steps_out) steps_out[q] = steps;
+            // fallback: non-finite state (e.g. a NaN or zero warm state that
+            // load_buffer could not normalise) or a growing residual
+            const bool bad = !std::isfinite(epa) || !std::isfinite(res);
+            const bool grew = have[q] && res > kFallbackGrowth * rmin[q];
+            if (!b->fallback[q].empty() && !b->fell_back[q] && (bad || grew)) {
+                if (gpe_load_buffer(h, b->fallback[q].data())) return bfail(b, h->err);
+                b->fell_back[q] = 1;
+                have[q] = 0;
+                keep.push_back(q);
+                continue;
+            }
+            if (e_out) e_out[q] = epa;
             const bool conv = tol > 0 && have[q] && std::abs(epa - last[q]) < tol * std::abs(epa);
             last[q] = epa;
+            rmin[q] = have[q] ? std::min(rmin[q], res) : res;
             have[q] = 1;
             if (!conv) keep.push_back(q);
         }
@@ -3750,6 +3935,8 @@ int gpe_batch_create(const gpe_params* params, int n, int precision, gpe_batch**
         h->own_stream = false;
         b->inst.push_back(h);
     }
+    b->fallback.assign(n, std::vector<double>());
+    b->fell_back.assign(n, 0);
     *out = b;
     return 0;
 }
@@ -3759,9 +3946,10 @@ void gpe_batch_destroy(gpe_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graphs(b);
     for (gpe_handle* h : b->inst) gpe_destroy(h);
-    void* bufs[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork};
+    void* bufs[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork, b->d_eargs, b->d_swork, b->d_ework, b->d_esum};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
+    if (b->h_eout) (void)hipHostFree(b->h_eout);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -3772,6 +3960,21 @@ gpe_handle* gpe_batch_instance(gpe_batch* b, int i) {
 }
 
 const char* gpe_batch_last_error(const gpe_batch* b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+int gpe_batch_set_fallback(gpe_batch* b, int i, const double* psi) {
+    if (!b || i < 0 || i >= (int)b->inst.size()) return bfail(b, "bad batch instance");
+    if (!psi) {
+        b->fallback[i].clear();
+        return 0;
+    }
+    b->fallback[i].assign(psi, psi + 2 * b->inst[i]->N);
+    return 0;
+}
+
+int gpe_batch_fallback_taken(const gpe_batch* b, int i) {
+    if (!b || i < 0 || i >= (int)b->fell_back.size()) return 0;
+    return b->fell_back[i] ? 1 : 0;
+}
 
 int gpe_batch_it_until(gpe_batch* b, long max_steps, long check_every, double tol, long* steps_out,
                        double* energy_per_atom_out) {

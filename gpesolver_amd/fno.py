@@ -143,10 +143,39 @@ def target(psi: np.ndarray) -> np.ndarray:
     return (a / a.max()).astype(np.float32)
 
 
-def warm_state(pred: np.ndarray, shape) -> np.ndarray:
+# Potentials even in x, y and z about the grid origin (index n/2 on each
+# axis: x_i = (i - n/2) d, src/context/context.cpp:17-33)
+EVEN_POTENTIALS = ("HARMONIC", "MEXICAN", "MEXICAN_FREE", "CYLINDRICAL", "FREE")
+
+
+def mirror_average(a: np.ndarray, axes=(0, 1, 2)) -> np.ndarray:
+    """Average of a with its mirror images x_i -> -x_i (index i -> n - i)
+    along each of `axes`, over the pairs that are both interior (2 <= i <=
+    n - 2).  Indices 0, 1 and n - 1 are kept: the image of the interior plane
+    1 is the boundary plane n - 1, which imaginary time holds at zero, so the
+    grid itself is not symmetric there."""
+    a = np.array(a, dtype=np.float64, copy=True)
+    for ax in axes:
+        b = np.moveaxis(a, ax, 0)
+        n = b.shape[0]
+        img = b.copy()
+        img[2:n - 1] = b[2:n - 1][::-1]
+        a = np.moveaxis(0.5 * (b + img), 0, ax)
+    return a
+
+
+def warm_state(pred: np.ndarray, shape, symmetric: bool = False) -> np.ndarray:
     """Initial state from a prediction: interior amplitude, boundary zero
-    (like the reference's generated initial states)."""
-    a = np.maximum(pred.reshape(shape).astype(np.float64), 0.0)
+    (like the reference's generated initial states).  symmetric: the
+    prediction is first averaged over the mirror images of an even potential,
+    the symmetry imaginary time keeps (the plain MULTIPLE_GAUSS start has it
+    too); a slightly asymmetric warm state would seed the unstable left / right
+    mode of a double well, which imaginary time then takes thousands of steps
+    to settle."""
+    a = pred.reshape(shape).astype(np.float64)
+    if symmetric:
+        a = mirror_average(a)
+    a = np.maximum(a, 0.0)
     psi = np.zeros(shape, dtype=np.complex128)
     psi[1:-1, 1:-1, 1:-1] = a[1:-1, 1:-1, 1:-1]
     return psi

@@ -25,6 +25,7 @@ EXPORTS = (
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_imag_time_until", "gpe_synchronize", "gpe_set_timing",
     "gpe_get_timing", "gpe_autotune",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
+    "gpe_batch_set_fallback", "gpe_batch_fallback_taken",
     "gpe_fno_create", "gpe_fno_destroy", "gpe_fno_last_error", "gpe_fno_num_params", "gpe_fno_get_params",
     "gpe_fno_set_params", "gpe_fno_get_grads", "gpe_fno_forward", "gpe_fno_loss_grad", "gpe_fno_adam_step",
     "gpe_gemm_test",
@@ -111,6 +112,10 @@ def load(path: str = LIB_PATH):
     L.gpe_batch_last_error.restype = C.c_char_p
     L.gpe_batch_it_until.argtypes = [vp, C.c_long, C.c_long, C.c_double, C.POINTER(C.c_long), dp]
     L.gpe_batch_it_until.restype = ip
+    L.gpe_batch_set_fallback.argtypes = [vp, ip, dp]
+    L.gpe_batch_set_fallback.restype = ip
+    L.gpe_batch_fallback_taken.argtypes = [vp, ip]
+    L.gpe_batch_fallback_taken.restype = ip
     L.gpes_validate_params_json.argtypes = [C.c_char_p, C.c_char_p, ip]
     L.gpes_validate_params_json.restype = ip
     L.gpes_run.argtypes = [C.c_char_p, C.c_double, dp, C.POINTER(C.c_long), C.c_char_p, ip]
@@ -306,6 +311,21 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_fallback(self, i: int, psi):
+        """State instance i restarts from if its own turns non-finite or its
+        residual grows (gpe_batch_set_fallback); None clears it."""
+        if psi is None:
+            rc = self.L.gpe_batch_set_fallback(self.b, i, None)
+        else:
+            a = np.ascontiguousarray(psi, dtype=np.complex128)
+            assert a.shape == self.instances[i].shape
+            rc = self.L.gpe_batch_set_fallback(self.b, i, _dp(a.view(np.float64)))
+        if rc != 0:
+            raise GpeError(self.L.gpe_batch_last_error(self.b).decode())
+
+    def fallbacks_taken(self) -> np.ndarray:
+        return np.array([bool(self.L.gpe_batch_fallback_taken(self.b, i)) for i in range(len(self.instances))])
 
     def it_until(self, max_steps: int, check_every: int = 100, tol: float = 0.0):
         n = len(self.instances)

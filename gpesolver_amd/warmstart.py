@@ -8,10 +8,14 @@ Pipeline:
      between 100-step checks (gpe_batch_it_until);
   3. train the FNO (gpesolver_amd.fno) on (features -> amplitude) with its own
      GPU forward / backward kernels and Adam;
-  4. warm start: psi0 = FNO prediction, then plain imaginary time refines it
-     to the same criterion.  Plain MULTIPLE_GAUSS imaginary time stays the
-     fallback (non-finite or empty prediction) and the acceptance check (the warm run
-     must reach the plain run's E/N).
+  4. warm start: psi0 = FNO prediction (averaged over the mirror images of
+     an even potential), then plain imaginary time refines it to the same
+     criterion.  Plain MULTIPLE_GAUSS imaginary time stays the fallback and
+     the acceptance check (the warm run must reach the plain run's E/N): a
+     non-finite or empty prediction is replaced before the run, and during the
+     run the batch restarts an instance from MULTIPLE_GAUSS when its state
+     turns non-finite or its residual ||H psi - mu psi|| grows
+     (gpe_batch_set_fallback).
 """
 from __future__ import annotations
 
@@ -54,21 +58,29 @@ def build(inst: dict, grid=GRID):
     return p, V, psi
 
 
-def run_batch(insts: list[dict], init_states, tol: float, max_steps: int, precision=hip.GPE_FP32, grid=GRID):
+def run_batch(insts: list[dict], init_states, tol: float, max_steps: int, precision=hip.GPE_FP32, grid=GRID,
+              fallback=False, with_taken=False):
     """Imaginary time on all instances together; returns (steps, E/N, final
-    states, wall seconds).  init_states[q] is the initial psi of instance q."""
+    states, wall seconds[, fallback taken per instance]).  init_states[q] is
+    the initial psi of instance q; fallback: the plain MULTIPLE_GAUSS state is
+    each instance's fallback (gpe_batch_set_fallback)."""
     built = [build(d, grid) for d in insts]
     b = hip.Batch([x[0].to_gpe() for x in built], precision)
     try:
         t0 = time.perf_counter()
-        for dev, (p, V, _), psi0 in zip(b.instances, built, init_states):
+        for q, (dev, (p, V, plain), psi0) in enumerate(zip(b.instances, built, init_states)):
             dev.load_pote(V)
             dev.load_buffer(psi0)
+            if fallback:
+                b.set_fallback(q, plain)
         steps, e = b.it_until(max_steps, 100, tol)
         wall = time.perf_counter() - t0
+        taken = b.fallbacks_taken()
         states = [d.export_psi() for d in b.instances]
     finally:
         b.close()
+    if with_taken:
+        return steps, e, states, wall, taken
     return steps, e, states, wall
 
 
@@ -120,7 +132,7 @@ def predict_states(model: F.Fno, insts: list[dict], grid=GRID, batch=8):
     for b0 in range(0, len(insts), batch):
         y = model.forward(feats[b0:b0 + batch])
         for q, d in enumerate(insts[b0:b0 + batch]):
-            psi = F.warm_state(y[q], grid)
+            psi = F.warm_state(y[q], grid, symmetric=d["pot"] in F.EVEN_POTENTIALS)
             if not np.all(np.isfinite(psi)) or np.abs(psi).max() <= 0.0:
                 psi = build(d, grid)[2]
                 fb.append(True)
@@ -137,7 +149,8 @@ def compare(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID
     t0 = time.perf_counter()
     warm_init, fb = predict_states(model, insts, grid)
     t_inf = time.perf_counter() - t0
-    s_warm, e_warm, _, t_warm = run_batch(insts, warm_init, tol, max_steps, grid=grid)
+    s_warm, e_warm, _, t_warm, taken = run_batch(insts, warm_init, tol, max_steps, grid=grid, fallback=True,
+                                                 with_taken=True)
     # acceptance check: the warm run must reach the plain run's energy
     de = np.abs(e_warm - e_plain) / np.abs(e_plain)
     return {
@@ -151,6 +164,7 @@ def compare(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID
         "plain_steps_max": int(np.max(s_plain)),
         "warm_steps_max": int(np.max(s_warm)),
         "fallbacks": int(fb.sum()),
+        "fallbacks_in_run": int(taken.sum()),
         "energy_rel_diff_max": float(de.max()),
         "energy_rel_diff_median": float(np.median(de)),
         "plain_steps": s_plain.tolist(),
@@ -184,7 +198,7 @@ def compare_single(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, gr
             out["plain_steps"].append(int(steps))
             t0 = time.perf_counter()
             y = model.forward(F.features(p, V, d["pot"])[None])
-            psi = F.warm_state(y[0], grid)
+            psi = F.warm_state(y[0], grid, symmetric=d["pot"] in F.EVEN_POTENTIALS)
             if not np.all(np.isfinite(psi)) or np.abs(psi).max() <= 0.0:
                 psi = P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2, sigma_cells=INIT_SIGMA_CELLS)
                 out["fallbacks"] += 1

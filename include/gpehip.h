@@ -129,6 +129,16 @@ const char *gpe_batch_last_error(const gpe_batch *b);
  * (length n, may be NULL) receive each instance's step count and final E/N. */
 int gpe_batch_it_until(gpe_batch *b, long max_steps, long check_every, double tol, long *steps_out,
                        double *energy_per_atom_out);
+/* Fallback of a warm-started instance (no reference counterpart; the
+ * fallback state enters through load_buffer, src/solver/solver.cpp:118-125):
+ * psi (interleaved complex128, the StdMat3D layout; NULL clears it) is the
+ * state gpe_batch_it_until restarts instance i from, once, when at a check
+ * its energy or residual is not finite (a NaN or zero state given to
+ * gpe_load_buffer) or its residual ||H psi - mu psi|| exceeds twice the
+ * smallest value it had so far.  Its step count keeps running.
+ * gpe_batch_fallback_taken tells whether the last gpe_batch_it_until took it. */
+int gpe_batch_set_fallback(gpe_batch *b, int i, const double *psi);
+int gpe_batch_fallback_taken(const gpe_batch *b, int i);
 
 /* Learned warm start (no reference counterpart): a Fourier neural operator
  * mapping per-point parameter fields [B][nx*ny*nz][cin] to a field

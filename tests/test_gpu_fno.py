@@ -63,3 +63,41 @@ def test_adam_reduces_loss():
         m.adam_step(3e-3)
     l1 = m.loss_grad(x, t)
     assert l1 < 0.5 * l0, (l0, l1)
+
+
+def test_config4_size_forward_loss_and_gradients():
+    """The warm-start network at its stated size (64x64x16, width 32, 4
+    spectral layers, 16 modes per axis capped at nz/2+1 = 9 along z, six
+    input channels): forward pass, loss and the largest-gradient entry of
+    every parameter group against the FP64 restatement (central finite
+    differences).  BF16 operands, so agreement is at the BF16 level."""
+    cfg = dict(nx=64, ny=64, nz=16, width=32, layers=4, hidden=64, modes=16, max_batch=1)
+    m = fno.Fno(**cfg, seed=21, cin=fno.CIN)
+    ref = FnoRef(64, 64, 16, fno.CIN, 32, 4, 64, 16)
+    assert m.num_params == ref.nparam == 18880993
+    rng = np.random.default_rng(21)
+    P = 64 * 64 * 16
+    x = (0.5 * rng.standard_normal((1, P, fno.CIN))).astype(np.float32)
+    xx, yy, zz = np.meshgrid(np.linspace(-1, 1, 64), np.linspace(-1, 1, 64), np.linspace(-1, 1, 16), indexing="ij")
+    t = np.exp(-(xx**2 + yy**2 + zz**2) * 3).reshape(1, -1).astype(np.float32)
+    w = m.get_params().astype(np.float64)
+    y = m.forward(x)
+    yr = ref.forward(w, x)
+    fwd = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    loss = m.loss_grad(x, t)
+    lr_ = ref.loss(w, x, t)
+    print(f"\nconfig-4 FNO: forward rel L2 {fwd:.2e}, loss {loss:.6e} vs {lr_:.6e}", flush=True)
+    assert fwd < 3e-2
+    assert abs(loss - lr_) < 3e-2 * lr_
+    g = m.get_grads()
+    idx = sorted({int(o + np.argmax(np.abs(g[o:o + n]))) for (o, n) in ref.off.values()})
+    gfd = ref.fd_grad(w, x, t, idx, eps=1e-3)
+    gg = g[idx].astype(np.float64)
+    cos = gg @ gfd / (np.linalg.norm(gg) * np.linalg.norm(gfd))
+    big = np.abs(gfd) > 0.05 * np.abs(gfd).max()
+    rel = np.abs(gg[big] - gfd[big]) / np.abs(gfd[big])
+    print(f"config-4 FNO gradients ({len(idx)} entries): cosine {cos:.5f}, median rel {np.median(rel):.2e}, "
+          f"max rel {rel.max():.2e}", flush=True)
+    assert cos > 0.99
+    assert np.median(rel) < 0.05 and rel.max() < 0.3
+    m.close()

@@ -30,3 +30,23 @@ def test_warm_start_needs_fewer_it_steps():
     # the warm run reaches the plain run's energy (or a lower one)
     pe, we = np.array(res["plain_e_per_atom"]), np.array(res["warm_e_per_atom"])
     assert np.all(we <= pe + 1e-4 * np.abs(pe))
+
+
+def test_fallback_on_nan_and_zero_warm_state():
+    """A NaN and a zero warm state, given to gpe_load_buffer (which cannot
+    normalise them), make the batch restart those instances from their
+    MULTIPLE_GAUSS fallback at the first check: they then reach the plain
+    run's E/N exactly, check_every steps later.  An instance with a good
+    warm state (here its own plain state) does not fall back."""
+    rng = np.random.default_rng(11)
+    insts = W.sample_instances(rng, 3)
+    plain = [W.build(d)[2] for d in insts]
+    s_p, e_p, _, _ = W.run_batch(insts, plain, tol=1e-6, max_steps=40000)
+    bad = [np.full(plain[0].shape, np.nan, dtype=np.complex128), np.zeros(plain[1].shape, dtype=np.complex128),
+           plain[2]]
+    s_w, e_w, _, _, taken = W.run_batch(insts, bad, tol=1e-6, max_steps=40000, fallback=True, with_taken=True)
+    assert s_p.max() < 39000  # all converge before the cap
+    print("plain", s_p, e_p, "fallback run", s_w, e_w, taken, flush=True)
+    assert taken.tolist() == [True, True, False]
+    np.testing.assert_array_equal(e_w, e_p)
+    np.testing.assert_array_equal(s_w, s_p + np.array([100, 100, 0]))
