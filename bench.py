@@ -32,7 +32,15 @@ Prints ONE JSON line.  Besides steps/s it reports:
   it_steps_per_s, it_time_to_converge
                 fused imaginary-time throughput on the same grid, and the
                 wall time of plain IT from MULTIPLE_GAUSS to |dE/N| < 1e-6
-                between 100-step checks (energies included).
+                between 100-step checks (energies included);
+  it_warm_vs_plain
+                the learned warm start on 64x64x16 (FNO width 32, 4 spectral
+                layers, 16 modes), trained in this run by its own kernels on
+                ground states the batched IT solver generates in this run
+                (the recipe of gpesolver_amd.warmstart_experiment, fewer
+                epochs), then batched IT to |dE/N| < 1e-6 on held-out
+                instances from the FNO's prediction against plain
+                MULTIPLE_GAUSS: wall times (inference included) and steps.
 """
 from __future__ import annotations
 
@@ -211,6 +219,41 @@ def host_cores() -> dict:
     return {"cpu_count": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
+def warm_vs_plain(n_train: int, n_test: int, epochs: int, seed: int = 20261203) -> dict:
+    """Train the warm-start FNO in this run and time batched IT, warm vs plain."""
+    from gpesolver_amd import fno as F
+    from gpesolver_amd import warmstart as W
+
+    rng = np.random.default_rng(seed)
+    insts = W.sample_instances(rng, n_train + n_test)
+    t0 = time.perf_counter()
+    X, T, S = W.generate(insts[:n_train], 1e-6, 5000, log=lambda m: None)
+    t_gen = time.perf_counter() - t0
+    model = F.Fno(*W.GRID, width=32, layers=4, hidden=64, modes=16, max_batch=8, seed=seed % 1000)
+    t0 = time.perf_counter()
+    hist = W.train(model, X, T, epochs, 1e-3, 8, log=lambda m: None)
+    t_train = time.perf_counter() - t0
+    del X, T
+    res = W.compare(model, insts[n_train:], 1e-6, 20000)
+    nparam = model.num_params
+    model.close()
+    ps, ws = np.array(res["plain_steps"]), np.array(res["warm_steps"])
+    return {
+        "grid": list(W.GRID), "instances": n_test, "tol_dE_per_N": 1e-6, "check_every": 100, "max_steps": 20000,
+        "plain_wall_s": res["plain_wall_s"], "warm_wall_s": res["warm_wall_s"],
+        "warm_inference_s": res["warm_inference_s"], "speedup_wall": res["speedup_wall"],
+        "plain_steps_mean": res["plain_steps_mean"], "warm_steps_mean": res["warm_steps_mean"],
+        "plain_steps_max": res["plain_steps_max"], "warm_steps_max": res["warm_steps_max"],
+        "step_ratio_median": float(np.median(ps / ws)),
+        "fallbacks_before_run": res["fallbacks"], "fallbacks_in_run": res["fallbacks_in_run"],
+        "energy_rel_diff_median": res["energy_rel_diff_median"],
+        "weights": f"trained in this run: {n_train} generated ground states, {epochs} epochs, batch 8, Adam lr 1e-3 "
+                   f"(cosine), {nparam} parameters, BF16 GEMM operands / FP32 accumulation",
+        "train_mse_last": hist[-1], "generate_s": t_gen, "train_s": t_train,
+        "train_steps_median": float(np.median(S)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +262,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64", action="store_true", help="--large: also time the FP64 path")
     ap.add_argument("--it-max-steps", type=int, default=20000, help="cap of the IT time-to-convergence leg")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg (trains the FNO, ~2 min)")
+    ap.add_argument("--warm-train", type=int, default=2000, help="warm-start leg: generated training states")
+    ap.add_argument("--warm-epochs", type=int, default=40, help="warm-start leg: training epochs")
     ap.add_argument("--rt-only", action="store_true",
                     help="profiling runs: the RT leg only, on the default launch shapes (split plane pass, "
                          "register x pass: what the autotuner picks at this grid), no autotune and no IT legs, "
@@ -314,6 +360,9 @@ def main():
                          "step_roofline": {"model_bytes_per_step": mb64,
                                            "achieved_GBps": mb64 * args.steps / el64 / 1e9,
                                            "frac": mb64 * args.steps / el64 / HBM_PEAK}}}
+
+    if not args.no_warm:
+        extra["it_warm_vs_plain"] = warm_vs_plain(args.warm_train, 64, args.warm_epochs)
 
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib << 20)

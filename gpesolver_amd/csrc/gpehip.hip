@@ -2195,6 +2195,101 @@ __global__ void k_energy_sums_group(const EArgs<T>* __restrict__ args, int pass)
     energy_sums(a.part, a.nx, pass, a.sums, a.out, a.m, a.N, a.dV, a.g, a.gamma, a.cdd, a.dv15, a.n25);
 }
 
+// ---------------------------------------------------------------------------
+// On-device initial states and potentials (src/initializer/initializer.cpp:46-248,
+// src/parameters/potentials.hpp:76-157).  Coordinates x_i = (i - n/2) d
+// (src/context/context.cpp:17-33); the potential sees them truncated toward
+// zero to whole atomic units (DataInitializer::_pote_func is a
+// std::function<double(int, int, int)>, src/initializer/initializer.hpp:37).
+constexpr int kMaxCenters = 64;
+struct InitArgs {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    int key;  // potential key (GPE_POT_*) or initial state (GPE_INIT_*)
+    // potential: m, b = m wx^2/2, aa = m wx^2/(4 dd^2), trap frequencies, slope
+    double m, b, aa, omx, omy, omz, slope;
+    // Gaussian sums: centres, widths, whether z is taken about the centre
+    int ncent, zcent;
+    double cx[kMaxCenters], cy[kMaxCenters], cz[kMaxCenters];
+    double sx, sy, sz;
+    double cos_dd, cos_r;  // COS: x offset dd, rrr = (nx/2) dx
+    double scale;          // sqrt(dV): the state is stored as phi = sqrt(dV) psi
+    double noise;          // phase noise amplitude (rad)
+    unsigned long long seed;
+};
+
+__device__ __forceinline__ double coord_of(int i, int n, double d) { return (double)(i - n / 2) * d; }
+
+template <typename T>
+__global__ void k_init_pote(InitArgs a, T* V) {
+    const size_t n = (size_t)a.nx * a.ny * a.nz;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % a.nz), j = (int)((e / a.nz) % a.ny), i = (int)(e / ((size_t)a.ny * a.nz));
+        const double x = trunc(coord_of(i, a.nx, a.dx)), y = trunc(coord_of(j, a.ny, a.dy)),
+                     z = trunc(coord_of(k, a.nz, a.dz));
+        const double vy = 0.5 * a.m * (y * y) * (a.omy * a.omy), vz = 0.5 * a.m * (z * z) * (a.omz * a.omz);
+        const double x2 = x * x, x4 = x2 * x2;
+        double v;
+        switch (a.key) {
+        case GPE_POT_HARMONIC: v = 0.5 * a.m * x2 * (a.omx * a.omx) + vy + vz; break;
+        case GPE_POT_MEXICAN: v = -a.b * x2 + a.aa * x4 + vy + vz; break;
+        case GPE_POT_MEXICAN_FREE: v = a.aa * x4 + vy + vz; break;
+        case GPE_POT_CYLINDRICAL: {
+            const double w = (a.omy + a.omx) / 2.;
+            v = 0.5 * a.m * (x2 + y * y) * (w * w) + vz;
+            break;
+        }
+        case GPE_POT_MEXICAN_ASYMETRIC: v = -a.b * x2 + a.aa * x4 - a.slope * x + vy + vz; break;
+        case GPE_POT_MEXICAN_ASYMETRIC_FREE: v = a.aa * x4 - a.slope * x + vy + vz; break;
+        default: v = vz; break;  // FREE
+        }
+        V[e] = (T)v;
+    }
+}
+
+// SplitMix64 (public-domain generator of Steele, Lea and Flood): the e-th
+// output of the stream that starts at `seed`, as a double in [0, 1).
+__device__ __forceinline__ double splitmix_u01(unsigned long long seed, unsigned long long e) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (e + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// psi_0 on the interior points (boundary zero), times the phase noise
+// exp(i noise (2u - 1)), stored as phi = sqrt(dV) psi into both buffers
+// (cpsi = cpsii = wvf, src/solver/solver.cpp:118-125).
+template <typename T>
+__global__ void k_init_psi(InitArgs a, cplx<T>* psi, cplx<T>* psii) {
+    const size_t n = (size_t)a.nx * a.ny * a.nz;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % a.nz), j = (int)((e / a.nz) % a.ny), i = (int)(e / ((size_t)a.ny * a.nz));
+        double val = 0.0;
+        if (i >= 1 && i <= a.nx - 2 && j >= 1 && j <= a.ny - 2 && k >= 1 && k <= a.nz - 2) {
+            const double x = coord_of(i, a.nx, a.dx), y = coord_of(j, a.ny, a.dy), z = coord_of(k, a.nz, a.dz);
+            if (a.key == GPE_INIT_COS) {
+                val = cos(2 * M_PI * (x - a.cos_dd) / a.cos_r) * cos(2 * M_PI * y / a.cos_r) * cos(1 * M_PI * z / a.cos_r);
+            } else {
+                for (int c = 0; c < a.ncent; ++c) {
+                    const double xc = x - a.cx[c], yc = y - a.cy[c], zc = a.zcent ? z - a.cz[c] : z;
+                    val += exp(-0.5 * (xc * xc) / (a.sx * a.sx)) * exp(-0.5 * (yc * yc) / (a.sy * a.sy)) *
+                           exp(-0.5 * (zc * zc) / (a.sz * a.sz));
+                }
+            }
+        }
+        double re = val, im = 0.0;
+        if (a.noise != 0.0) {
+            const double th = a.noise * (2.0 * splitmix_u01(a.seed, e) - 1.0);
+            re = val * cos(th);
+            im = val * sin(th);
+        }
+        const cplx<T> c = gpe::mk((T)(re * a.scale), (T)(im * a.scale));
+        psi[e] = c;
+        psii[e] = c;
+    }
+}
+
 // Scratch of gpe_calc_energy_noalloc: per-plane partials (8 doubles per
 // plane) and the pass-0 sums.  Host code never uses this name as an address:
 // it takes the per-device address from hipGetSymbolAddress (energy_scratch).
@@ -3699,6 +3794,35 @@ static int do_it_until(gpe_handle* h, long max_steps, long check_every, double t
     return 0;
 }
 
+static InitArgs init_args(const gpe_handle* h) {
+    InitArgs a;
+    memset(&a, 0, sizeof(a));
+    a.nx = h->p.nx;
+    a.ny = h->p.ny;
+    a.nz = h->p.nz;
+    a.dx = h->p.dx;
+    a.dy = h->p.dy;
+    a.dz = h->p.dz;
+    a.m = h->p.m;
+    a.scale = h->sqdv;
+    return a;
+}
+
+template <typename T>
+static int do_init_pote(gpe_handle* h, const InitArgs& a) {
+    hipLaunchKernelGGL((k_init_pote<T>), dim3(1024), dim3(256), 0, h->stream, a, (T*)h->V);
+    GPE_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int do_init_psi(gpe_handle* h, const InitArgs& a) {
+    hipLaunchKernelGGL((k_init_psi<T>), dim3(1024), dim3(256), 0, h->stream, a, (cplx<T>*)h->store[h->cur],
+                       (cplx<T>*)h->psii);
+    GPE_CHECK(hipGetLastError());
+    return 0;
+}
+
 #define DISPATCH(h, fn, ...) ((h)->prec == GPE_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
 extern "C" {
@@ -3868,6 +3992,94 @@ int gpe_import_fi3d(gpe_handle* h, const double* fi3d) {
     if (!h || !fi3d) return fail(h, "null argument");
     if (DISPATCH(h, flush_rt, h)) return 1;
     return upload_real(h, h->phi, fi3d, h->N, 1.0);
+}
+
+int gpe_init_potential(gpe_handle* h, int key, const gpe_trap* t) {
+    if (!h || !t) return fail(h, "null argument");
+    if (key < GPE_POT_HARMONIC || key > GPE_POT_MEXICAN_ASYMETRIC_FREE) return fail(h, "unknown potential key");
+    if (DISPATCH(h, flush_rt, h)) return 1;  // a pending half phase uses the old V
+    InitArgs a = init_args(h);
+    a.key = key;
+    // parameters.cpp:37-41 (aa, b) and potentials.hpp:121-157 (the slope)
+    a.omx = t->omega_x;
+    a.omy = t->omega_y;
+    a.omz = t->omega_z;
+    a.b = 0.5 * h->p.m * pow(t->omega_x, 2);
+    a.aa = h->p.m * pow(t->omega_x, 2) / 4. / pow(t->dd, 2);
+    a.slope = (0.25 / (27211.4 * 11.6 * 1e9)) / t->dd;
+    return DISPATCH(h, do_init_pote, h, a);
+}
+
+int gpe_init_wavefunction(gpe_handle* h, int strategy, const gpe_trap* t, int n_gauss_max, int bx, int by, int bz,
+                          double sigma_cells, double phase_noise, unsigned long long seed) {
+    if (!h || !t) return fail(h, "null argument");
+    const gpe_params& p = h->p;
+    InitArgs a = init_args(h);
+    a.key = strategy;
+    a.noise = phase_noise;
+    a.seed = seed;
+    const double Lx = p.nx * p.dx, Ly = p.ny * p.dy, Lz = p.nz * p.dz;
+    auto add = [&](double x, double y, double z) -> bool {
+        if (a.ncent >= kMaxCenters) return false;
+        a.cx[a.ncent] = x;
+        a.cy[a.ncent] = y;
+        a.cz[a.ncent] = z;
+        ++a.ncent;
+        return true;
+    };
+    switch (strategy) {
+    case GPE_INIT_COS:  // initializer.cpp:46-74
+        a.cos_dd = t->dd;
+        a.cos_r = (p.nx / 2) * p.dx;
+        break;
+    case GPE_INIT_GAUSS:  // initializer.cpp:76-99
+        a.sx = sigma_cells > 0 ? sigma_cells * p.dx : Lx / 10.;
+        a.sy = sigma_cells > 0 ? sigma_cells * p.dy : Ly / 10.;
+        a.sz = sigma_cells > 0 ? sigma_cells * p.dz : Lz / 10.;
+        add(0, 0, 0);
+        break;
+    case GPE_INIT_MULTIPLE_GAUSS:  // initializer.cpp:162-219
+        if (n_gauss_max <= 0) return fail(h, "Number of Gaussian maxima must be positive.");
+        a.sx = sigma_cells > 0 ? sigma_cells * p.dx : Lx / 20.;
+        a.sy = sigma_cells > 0 ? sigma_cells * p.dy : Ly / 20.;
+        a.sz = sigma_cells > 0 ? sigma_cells * p.dz : Lz / 20.;
+        for (int idx = 0; idx < n_gauss_max; ++idx) {
+            const double cx = idx % 2 ? t->dd : -t->dd;
+            double cy;
+            if (n_gauss_max % 2 == 1)
+                cy = (idx - n_gauss_max / 2) * (p.ny * p.dy) / 2. / (n_gauss_max + 1.);
+            else
+                cy = (double)(0 - p.ny / 2) * p.dy + (idx / 2 + 1) * (p.ny * p.dy) / (n_gauss_max / 2 + 1);
+            if (!add(cx, cy, 0)) return fail(h, "too many Gaussian maxima for the device initializer");
+        }
+        break;
+    case GPE_INIT_SETUP_GAUSS:  // initializer.cpp:101-160
+        if (bx <= 0 || by <= 0 || bz <= 0) return fail(h, "droplet counts must be positive");
+        a.sx = Lx / 20.;
+        a.sy = Ly / 20.;
+        a.sz = Lz / 20.;
+        a.zcent = 1;
+        for (int ia = 0; ia < bx; ++ia)
+            for (int ib = 0; ib < by; ++ib)
+                for (int ic = 0; ic < bz; ++ic)
+                    if (!add(-Lx / 2 + Lx / (bx + 1) * (ia + 1), -Ly / 2 + Ly / (by + 1) * (ib + 1),
+                             -Lz / 2 + Lz / (bz + 1) * (ic + 1)))
+                        return fail(h, "too many droplets for the device initializer");
+        break;
+    default:
+        return fail(h, "unknown initial-state strategy");
+    }
+    if (DISPATCH(h, do_init_psi, h, a)) return 1;
+    h->norm_pending = false;
+    h->rt_pending = false;  // the new state replaces a pending RT step
+    // calc_norm + normalize (solver.cpp:118-125)
+    if (DISPATCH(h, do_calc_norm, h, nullptr)) return 1;
+    return DISPATCH(h, do_normalize, h);
+}
+
+int gpe_export_pote(gpe_handle* h, double* V) {
+    if (!h || !V) return fail(h, "null argument");
+    return download_real(h, V, h->V, h->N, 1.0);
 }
 
 int gpe_set_coupling(gpe_handle* h, double ggp11, double gamma) {

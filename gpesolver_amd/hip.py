@@ -20,7 +20,7 @@ GPE_FP64 = 64
 EXPORTS = (
     "gpe_create", "gpe_destroy", "gpe_last_error", "gpe_version",
     "gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d",
-    "gpe_set_coupling", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
+    "gpe_set_coupling", "gpe_init_potential", "gpe_init_wavefunction", "gpe_export_pote", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy", "gpe_calc_energy_noalloc",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_imag_time_until", "gpe_synchronize", "gpe_set_timing",
     "gpe_get_timing", "gpe_autotune",
@@ -52,6 +52,16 @@ class GpeEnergies(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class GpeTrap(C.Structure):
+    _fields_ = [("omega_x", C.c_double), ("omega_y", C.c_double), ("omega_z", C.c_double), ("dd", C.c_double)]
+
+
+# potential keys and initial states (include/gpehip.h, GPE_POT_* / GPE_INIT_*)
+POT_KEYS = {"HARMONIC": 0, "MEXICAN": 1, "MEXICAN_FREE": 2, "CYLINDRICAL": 3, "FREE": 4, "MEXICAN_ASYMETRIC": 5,
+            "MEXICAN_ASYMETRIC_FREE": 6}
+INIT_KEYS = {"COS": 0, "GAUSS": 1, "MULTIPLE_GAUSS": 2, "SETUP_GAUSS": 3}
+
+
 class GpeError(RuntimeError):
     pass
 
@@ -75,7 +85,13 @@ def load(path: str = LIB_PATH):
     L.gpe_last_error.argtypes = [vp]
     L.gpe_last_error.restype = C.c_char_p
     L.gpe_version.restype = C.c_char_p
-    for name in ("gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d"):
+    L.gpe_init_potential.argtypes = [vp, ip, C.POINTER(GpeTrap)]
+    L.gpe_init_potential.restype = ip
+    L.gpe_init_wavefunction.argtypes = [vp, ip, C.POINTER(GpeTrap), ip, ip, ip, ip, C.c_double, C.c_double,
+                                        C.c_ulonglong]
+    L.gpe_init_wavefunction.restype = ip
+    for name in ("gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d",
+                 "gpe_export_pote"):
         getattr(L, name).argtypes = [vp, dp]
         getattr(L, name).restype = ip
     L.gpe_set_coupling.argtypes = [vp, C.c_double, C.c_double]
@@ -213,6 +229,26 @@ class Device:
 
     def kinetic_step(self):
         self._check(self.L.gpe_real_kinetic_step(self.h))
+
+    # on-device initial state and potential (gpe_init_potential / gpe_init_wavefunction)
+    @staticmethod
+    def trap_of(p) -> GpeTrap:
+        return GpeTrap(p.omega_x, p.omega_y, p.omega_z, p.dd)
+
+    def init_potential(self, key: str, p):
+        """V of potential `key` for the parameters p (PhysicalParameters), on the device."""
+        self._check(self.L.gpe_init_potential(self.h, POT_KEYS[key], C.byref(self.trap_of(p))))
+
+    def init_wavefunction(self, strategy: str, p, n_gauss_max: int = 2, drops=(3, 3, 2), sigma_cells: float = 0.0,
+                          phase_noise: float = 0.0, seed: int = 0):
+        """psi_0 of `strategy` on the device, then load_buffer's norm and normalize."""
+        self._check(self.L.gpe_init_wavefunction(self.h, INIT_KEYS[strategy], C.byref(self.trap_of(p)), n_gauss_max,
+                                                 drops[0], drops[1], drops[2], sigma_cells, phase_noise, seed))
+
+    def export_pote(self) -> np.ndarray:
+        V = np.empty(self.shape, dtype=np.float64)
+        self._check(self.L.gpe_export_pote(self.h, _dp(V)))
+        return V
 
     def energy(self) -> dict:
         e = GpeEnergies()

@@ -674,6 +674,47 @@ void HipGrossPitaevskiSolver::import_data() {
 }
 void HipGrossPitaevskiSolver::import_pote() { check(gpe_load_pote(h, pote.data())); }
 
+static gpe_trap trap_of(const PhysicalParameters& p) {
+    gpe_trap t;
+    t.omega_x = p.omega_x;
+    t.omega_y = p.omega_y;
+    t.omega_z = p.omega_z;
+    t.dd = p.dd;
+    return t;
+}
+
+void HipGrossPitaevskiSolver::init_state_on_device(const PhysicalParameters& p) {
+    int strategy;
+    switch (p.init_strategy) {
+    case InitStrategy::COS: strategy = GPE_INIT_COS; break;
+    case InitStrategy::GAUSS: strategy = GPE_INIT_GAUSS; break;
+    case InitStrategy::MULTIPLE_GAUSS: strategy = GPE_INIT_MULTIPLE_GAUSS; break;
+    case InitStrategy::SETUP_GAUSS: strategy = GPE_INIT_SETUP_GAUSS; break;
+    default: throw std::runtime_error("init_state_on_device: file strategies load through FileManager");
+    }
+    const gpe_trap t = trap_of(p);
+    // the host initializer applies gauss_sigma_cells to GAUSS only
+    const double sigma = p.init_strategy == InitStrategy::GAUSS ? p.gauss_sigma_cells : 0.0;
+    check(gpe_init_wavefunction(h, strategy, &t, p.n_gauss_max, p.bec_droplets_x, p.bec_droplets_y, p.bec_droplets_z,
+                                sigma, p.phase_noise, p.noise_seed));
+    cpsi.assign((size_t)p.nx * p.ny * p.nz, {0.0, 0.0});
+    check(gpe_export_psi(h, reinterpret_cast<double*>(cpsi.data())));
+    cpsii = cpsi;
+}
+
+void HipGrossPitaevskiSolver::init_pote_on_device(const std::string& key, potential_t& V) {
+    static const char* keys[] = {"HARMONIC", "MEXICAN", "MEXICAN_FREE", "CYLINDRICAL", "FREE", "MEXICAN_ASYMETRIC",
+                                 "MEXICAN_ASYMETRIC_FREE"};
+    int k = GPE_POT_FREE;  // an unknown key falls back to FREE (potentials.hpp:48,53)
+    for (int q = 0; q < 7; ++q)
+        if (key == keys[q]) k = q;
+    const gpe_trap t = trap_of(*params);
+    check(gpe_init_potential(h, k, &t));
+    V.assign((size_t)params->nx * params->ny * params->nz, 0.0);
+    check(gpe_export_pote(h, V.data()));
+    pote = V;
+}
+
 void HipGrossPitaevskiSolver::imag_time_iters(long n) {
     check(gpe_set_coupling(h, params->ggp11, params->gamma));
     check(gpe_imag_time_iters(h, n));
@@ -720,18 +761,30 @@ void SimulationManager::run_simulation() { m_solver->solve(); }
 void SimulationManager::on_params_loaded() {
     params.init_parameters();
     m_solver->initialize();
+    auto* dev = dynamic_cast<HipGrossPitaevskiSolver*>(m_solver.get());
     wavefunction_t psi;
-    if (params.init_strategy == InitStrategy::FROM_BINARY_FILE)
+    if (params.init_strategy == InitStrategy::FROM_BINARY_FILE) {
         request_load_from_binary(psi);
-    else if (params.init_strategy == InitStrategy::FROM_TEXT_FILE)
+    } else if (params.init_strategy == InitStrategy::FROM_TEXT_FILE) {
         request_load_from_text(psi);
-    else
+    } else if (dev) {
+        dev->init_state_on_device(params);  // generated on the device
+    } else {
         init_wavefunction(params, psi);
-    add_phase_noise(psi, params.phase_noise, params.noise_seed);
-    on_data_initialized(psi);
+    }
+    if (!psi.empty() || !dev) {
+        add_phase_noise(psi, params.phase_noise, params.noise_seed);
+        on_data_initialized(psi);
+    }
     potential_t V;
-    fill_potential(params.pote_key, params, V);
-    on_pote_initialized(V);
+    fill_potential(params.pote_key, params, V);  // FP64 values for the file the reference writes
+    if (dev) {
+        potential_t Vd;
+        dev->init_pote_on_device(params.pote_key, Vd);
+        m_file_manager->save_pote_to_text_file(V, "initial_potential");
+    } else {
+        on_pote_initialized(V);
+    }
 }
 
 void SimulationManager::on_data_loaded(const wavefunction_t& w) { m_solver->load_buffer(w); }
@@ -744,7 +797,13 @@ void SimulationManager::on_pote_initialized(const potential_t& V) {
 void SimulationManager::change_potential(const std::string& key) {
     params.pote_key = key;
     potential_t V;
-    fill_potential(key, params, V);
+    fill_potential(key, params, V);  // FP64 values for the file the reference writes
+    if (auto* dev = dynamic_cast<HipGrossPitaevskiSolver*>(m_solver.get())) {
+        potential_t Vd;
+        dev->init_pote_on_device(key, Vd);
+        m_file_manager->save_pote_to_text_file(V, "changed_potential");
+        return;
+    }
     m_file_manager->save_pote_to_text_file(V, "changed_potential");
     m_solver->load_pote(V);
 }

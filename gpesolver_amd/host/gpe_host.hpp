@@ -216,6 +216,13 @@ class HipGrossPitaevskiSolver : public AbstractGrossPitaevskiSolver {
 public:
     using AbstractGrossPitaevskiSolver::AbstractGrossPitaevskiSolver;
     ~HipGrossPitaevskiSolver() override;
+    // The generated initial states (COS, GAUSS, MULTIPLE_GAUSS, SETUP_GAUSS,
+    // with the phase noise) and the potentials built on the device
+    // (gpe_init_wavefunction / gpe_init_potential) instead of on the host:
+    // load_buffer / load_pote semantics, the host copies (cpsi, pote) are
+    // read back for the files the reference writes.
+    void init_state_on_device(const PhysicalParameters& p);
+    void init_pote_on_device(const std::string& key, potential_t& V);
 
 protected:
     gpe_handle* h = nullptr;

@@ -62,15 +62,20 @@ def run_batch(insts: list[dict], init_states, tol: float, max_steps: int, precis
               fallback=False, with_taken=False):
     """Imaginary time on all instances together; returns (steps, E/N, final
     states, wall seconds[, fallback taken per instance]).  init_states[q] is
-    the initial psi of instance q; fallback: the plain MULTIPLE_GAUSS state is
-    each instance's fallback (gpe_batch_set_fallback)."""
+    the initial psi of instance q, or None for the plain MULTIPLE_GAUSS start;
+    potentials and plain starts are built on the device
+    (gpe_init_potential / gpe_init_wavefunction).  fallback: the plain
+    MULTIPLE_GAUSS state is each instance's fallback (gpe_batch_set_fallback)."""
     built = [build(d, grid) for d in insts]
     b = hip.Batch([x[0].to_gpe() for x in built], precision)
     try:
         t0 = time.perf_counter()
-        for q, (dev, (p, V, plain), psi0) in enumerate(zip(b.instances, built, init_states)):
-            dev.load_pote(V)
-            dev.load_buffer(psi0)
+        for q, (dev, d, (p, V, plain), psi0) in enumerate(zip(b.instances, insts, built, init_states)):
+            dev.init_potential(d["pot"], p)
+            if psi0 is None:
+                dev.init_wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2, sigma_cells=INIT_SIGMA_CELLS)
+            else:
+                dev.load_buffer(psi0)
             if fallback:
                 b.set_fallback(q, plain)
         steps, e = b.it_until(max_steps, 100, tol)
@@ -89,8 +94,7 @@ def generate(insts: list[dict], tol=1e-6, max_steps=5000, chunk=64, log=print, g
     X, T, S = [], [], []
     for c0 in range(0, len(insts), chunk):
         part = insts[c0:c0 + chunk]
-        inits = [build(d, grid)[2] for d in part]
-        steps, e, states, wall = run_batch(part, inits, tol, max_steps, grid=grid)
+        steps, e, states, wall = run_batch(part, [None] * len(part), tol, max_steps, grid=grid)
         for d, psi in zip(part, states):
             p, V, _ = build(d, grid)
             X.append(F.features(p, V, d["pot"]))
@@ -144,8 +148,7 @@ def predict_states(model: F.Fno, insts: list[dict], grid=GRID, batch=8):
 
 def compare(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID):
     """Plain MULTIPLE_GAUSS vs warm-started batched IT on the same GPU."""
-    plain_init = [build(d, grid)[2] for d in insts]
-    s_plain, e_plain, _, t_plain = run_batch(insts, plain_init, tol, max_steps, grid=grid)
+    s_plain, e_plain, _, t_plain = run_batch(insts, [None] * len(insts), tol, max_steps, grid=grid)
     t0 = time.perf_counter()
     warm_init, fb = predict_states(model, insts, grid)
     t_inf = time.perf_counter() - t0

@@ -77,6 +77,42 @@ int gpe_import_fi3d(gpe_handle *h, const double *fi3d);
  * new contact and LHY couplings for the following steps. */
 int gpe_set_coupling(gpe_handle *h, double ggp11, double gamma);
 
+/* On-device initial states and potentials, per handle (and so per batch
+ * instance).  Keys in the reference's registration order
+ * (src/parameters/potentials.hpp:76-157, src/parameters/complex_param.hpp). */
+enum {
+    GPE_POT_HARMONIC = 0,
+    GPE_POT_MEXICAN = 1,
+    GPE_POT_MEXICAN_FREE = 2,
+    GPE_POT_CYLINDRICAL = 3,
+    GPE_POT_FREE = 4,
+    GPE_POT_MEXICAN_ASYMETRIC = 5,
+    GPE_POT_MEXICAN_ASYMETRIC_FREE = 6
+};
+enum { GPE_INIT_COS = 0, GPE_INIT_GAUSS = 1, GPE_INIT_MULTIPLE_GAUSS = 2, GPE_INIT_SETUP_GAUSS = 3 };
+/* Trap of PhysicalParameters (src/parameters/parameters.hpp:10-123), a.u. */
+typedef struct {
+    double omega_x, omega_y, omega_z;
+    double dd;
+} gpe_trap;
+/* DataInitializer::init_pote (src/initializer/initializer.cpp:229-248):
+ * V on every grid point, computed on the device.  Replaces building V on the
+ * host and gpe_load_pote. */
+int gpe_init_potential(gpe_handle *h, int pote_key, const gpe_trap *trap);
+/* DataInitializer::init_wavefunction for COS / GAUSS / MULTIPLE_GAUSS /
+ * SETUP_GAUSS (src/initializer/initializer.cpp:46-219) on the device, then
+ * load_buffer (src/solver/solver.cpp:118-125): cpsi = cpsii = psi_0, norm,
+ * normalize.  sigma_cells > 0 sets the GAUSS / MULTIPLE_GAUSS width to that
+ * many cells per axis (the reference's L/10 and L/20 otherwise);
+ * phase_noise > 0 multiplies psi_0 by exp(i phase_noise (2u - 1)), u the
+ * e-th SplitMix64 output of noise_seed at layout index e. */
+int gpe_init_wavefunction(gpe_handle *h, int init_strategy, const gpe_trap *trap, int n_gauss_max, int drops_x,
+                          int drops_y, int drops_z, double sigma_cells, double phase_noise,
+                          unsigned long long noise_seed);
+/* The potential as the solver holds it (the reference keeps it in
+ * CPUSolverData::pote, src/solver/solver_data/cpu_solver_data.hpp:11-31). */
+int gpe_export_pote(gpe_handle *h, double *V);
+
 /* Hooks (src/solver/solver.hpp:55-63). */
 int gpe_calc_fi3d(gpe_handle *h);                   /* calc_fi3d: fftw_poisson_solver.cpp:51-96 */
 int gpe_calc_norm(gpe_handle *h, double *xnorma);   /* calc_norm: cpu_solver.cpp:19-34 */
