@@ -2413,6 +2413,16 @@ struct gpe_handle {
     bool rt_pending = false;
     int rt_par = 0;
     double rt_pend[2] = {0, 0};  // scaled couplings (g', gamma') of that half phase
+    // Snapshots (gpe_snapshot_begin / _end): the state and its energies are
+    // copied on the device behind the launched work, then to pinned host
+    // memory on a second stream, so the loop keeps running meanwhile.
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ev = nullptr, copy_ev = nullptr;
+    void* snap = nullptr;           // device copy of the state (bulk precision, phi units)
+    double* snap_dev = nullptr;     // device: sums[8], energies[8]
+    void* snap_host = nullptr;      // pinned host copy of the state
+    double* snap_host_e = nullptr;  // pinned host copy of the energies
+    bool snap_in_flight = false;
     int* cbase = nullptr;        // device step counter read by graph nodes
     hipGraphExec_t rt_graph = nullptr;
     hipGraphExec_t it_graph[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -3332,6 +3342,28 @@ static int do_energy(gpe_handle* h, gpe_energies* out) {
 }
 
 
+// The energies of the materialized state enqueued on the handle's stream:
+// the two k_energy passes (partials in part, nx * 8 doubles), their sums in
+// sums (8 doubles; pass 1 reads mu from sums[5]) and the eight values, in
+// gpe_energies order, to out_dev.  No host copy, no synchronisation.
+template <typename T>
+static int enqueue_energy(gpe_handle* h, double* part, double* sums, double* out_dev) {
+    const gpe_params& p = h->p;
+    const double dV = p.dx * p.dy * p.dz;
+    const double gq = (h->ggp11 - p.cdd / 3) * p.n_atoms / dV, gmq = h->gamma * p.w_15 / pow(dV, 1.5);
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL((k_energy<T, 256>), dim3(p.nx), dim3(256), 0, h->stream, (const cplx<T>*)h->store[h->cur],
+                           (const T*)h->V, (const T*)h->phi, p.nx, p.ny, p.nz, 1.0 / (p.dx * p.dx),
+                           1.0 / (p.dy * p.dy), 1.0 / (p.dz * p.dz), p.m, p.cdd, gq, gmq, 0.0, pass, part,
+                           pass ? sums + 5 : nullptr);
+        GPE_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_energy_sums, dim3(1), dim3(64), 0, h->stream, part, p.nx, pass, sums, out_dev, p.m,
+                           p.n_atoms, dV, h->ggp11, h->gamma, p.cdd, pow(dV, 1.5), pow(p.n_atoms, 2.5));
+        GPE_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
 // gpe_calc_energy without allocation, host copies or synchronisation: the
 // partials and sums live in the device's `scratch`, the outputs go to out_dev.
 template <typename T>
@@ -3343,19 +3375,59 @@ static int do_energy_noalloc(gpe_handle* h, double* out_dev) {
     double* sc = energy_scratch(h->dev);
     if (!sc) return fail(h, "gpe_calc_energy_noalloc: no scratch address for this device");
     if (materialize<T>(h)) return 1;
-    const double dV = p.dx * p.dy * p.dz;
-    const double gq = (h->ggp11 - p.cdd / 3) * p.n_atoms / dV, gmq = h->gamma * p.w_15 / pow(dV, 1.5);
-    double* part = sc;
-    double* sums = sc + (size_t)kNoAllocPlanes * 8;
-    for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL((k_energy<T, 256>), dim3(p.nx), dim3(256), 0, h->stream, (const cplx<T>*)h->store[h->cur],
-                           (const T*)h->V, (const T*)h->phi, p.nx, p.ny, p.nz, 1.0 / (p.dx * p.dx),
-                           1.0 / (p.dy * p.dy), 1.0 / (p.dz * p.dz), p.m, p.cdd, gq, gmq, 0.0, pass, part,
-                           pass ? sums + 5 : nullptr);
-        GPE_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_energy_sums, dim3(1), dim3(64), 0, h->stream, part, p.nx, pass, sums, out_dev, p.m,
-                           p.n_atoms, dV, h->ggp11, h->gamma, p.cdd, pow(dV, 1.5), pow(p.n_atoms, 2.5));
-        GPE_CHECK(hipGetLastError());
+    return enqueue_energy<T>(h, sc, sc + (size_t)kNoAllocPlanes * 8, out_dev);
+}
+
+// Snapshot of the state and its energies, behind the launched work: the
+// state is copied on the device and its energies computed on the handle's
+// stream; a second stream then copies both to pinned host memory once that
+// point is reached, while the handle's stream goes on with later work.
+template <typename T>
+static int do_snapshot_begin(gpe_handle* h) {
+    if (h->snap_in_flight) return fail(h, "gpe_snapshot_begin: a snapshot is already in flight");
+    const size_t bytes = h->N * csize(h);
+    if (!h->copy_stream) {
+        GPE_CHECK(hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+        GPE_CHECK(hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming));
+        GPE_CHECK(hipEventCreateWithFlags(&h->copy_ev, hipEventDisableTiming));
+        GPE_CHECK(hipMalloc(&h->snap, bytes));
+        GPE_CHECK(hipMalloc((void**)&h->snap_dev, 16 * sizeof(double)));
+        GPE_CHECK(hipHostMalloc(&h->snap_host, bytes, hipHostMallocDefault));
+        GPE_CHECK(hipHostMalloc((void**)&h->snap_host_e, 8 * sizeof(double), hipHostMallocDefault));
+    }
+    if (materialize<T>(h)) return 1;
+    if (enqueue_energy<T>(h, h->epart, h->snap_dev, h->snap_dev + 8)) return 1;
+    GPE_CHECK(hipMemcpyAsync(h->snap, h->store[h->cur], bytes, hipMemcpyDeviceToDevice, h->stream));
+    GPE_CHECK(hipEventRecord(h->snap_ev, h->stream));
+    GPE_CHECK(hipStreamWaitEvent(h->copy_stream, h->snap_ev, 0));
+    GPE_CHECK(hipMemcpyAsync(h->snap_host, h->snap, bytes, hipMemcpyDeviceToHost, h->copy_stream));
+    GPE_CHECK(hipMemcpyAsync(h->snap_host_e, h->snap_dev + 8, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                             h->copy_stream));
+    GPE_CHECK(hipEventRecord(h->copy_ev, h->copy_stream));
+    h->snap_in_flight = true;
+    return 0;
+}
+
+template <typename T>
+static int do_snapshot_end(gpe_handle* h, double* psi, gpe_energies* e) {
+    if (!h->snap_in_flight) return fail(h, "gpe_snapshot_end: no snapshot in flight");
+    GPE_CHECK(hipEventSynchronize(h->copy_ev));
+    h->snap_in_flight = false;
+    if (psi) {
+        const T* src = (const T*)h->snap_host;
+        const double sc = 1.0 / h->sqdv;
+        for (size_t t = 0; t < 2 * h->N; ++t) psi[t] = (double)src[t] * sc;
+    }
+    if (e) {
+        const double* v = h->snap_host_e;
+        e->e_kin = v[0];
+        e->e_pot = v[1];
+        e->e_int = v[2];
+        e->e_ext = v[3];
+        e->e_bmf = v[4];
+        e->e_total = v[5];
+        e->mu = v[6];
+        e->residual = v[7];
     }
     return 0;
 }
@@ -3954,6 +4026,16 @@ void gpe_destroy(gpe_handle* h) {
     if (h->coef_host) (void)hipHostFree(h->coef_host);
     if (h->coef_ev) (void)hipEventDestroy(h->coef_ev);
     if (h->cbase) (void)hipFree(h->cbase);
+    if (h->copy_stream) {
+        (void)hipStreamSynchronize(h->copy_stream);
+        (void)hipStreamDestroy(h->copy_stream);
+    }
+    if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
+    if (h->copy_ev) (void)hipEventDestroy(h->copy_ev);
+    if (h->snap) (void)hipFree(h->snap);
+    if (h->snap_dev) (void)hipFree(h->snap_dev);
+    if (h->snap_host) (void)hipHostFree(h->snap_host);
+    if (h->snap_host_e) (void)hipHostFree(h->snap_host_e);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -4098,6 +4180,10 @@ int gpe_real_kinetic_step(gpe_handle* h) { return h ? DISPATCH(h, do_kinetic, h)
 int gpe_calc_energy(gpe_handle* h, gpe_energies* out) {
     if (!h || !out) return fail(h, "null argument");
     return DISPATCH(h, do_energy, h, out);
+}
+int gpe_snapshot_begin(gpe_handle* h) { return h ? DISPATCH(h, do_snapshot_begin, h) : 1; }
+int gpe_snapshot_end(gpe_handle* h, double* psi, gpe_energies* e) {
+    return h ? DISPATCH(h, do_snapshot_end, h, psi, e) : 1;
 }
 int gpe_calc_energy_noalloc(gpe_handle* h, double* out_dev) {
     if (!h || !out_dev) return fail(h, "null argument");

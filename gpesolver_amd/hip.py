@@ -22,6 +22,7 @@ EXPORTS = (
     "gpe_load_buffer", "gpe_load_pote", "gpe_export_psi", "gpe_export_fi3d", "gpe_import_fi3d",
     "gpe_set_coupling", "gpe_init_potential", "gpe_init_wavefunction", "gpe_export_pote", "gpe_calc_fi3d", "gpe_calc_norm", "gpe_normalize", "gpe_imag_iter_step",
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy", "gpe_calc_energy_noalloc",
+    "gpe_snapshot_begin", "gpe_snapshot_end",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_imag_time_until", "gpe_synchronize", "gpe_set_timing",
     "gpe_get_timing", "gpe_autotune",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
@@ -106,6 +107,10 @@ def load(path: str = LIB_PATH):
     L.gpe_calc_energy.restype = ip
     L.gpe_calc_energy_noalloc.argtypes = [vp, vp]
     L.gpe_calc_energy_noalloc.restype = ip
+    L.gpe_snapshot_begin.argtypes = [vp]
+    L.gpe_snapshot_begin.restype = ip
+    L.gpe_snapshot_end.argtypes = [vp, dp, C.POINTER(GpeEnergies)]
+    L.gpe_snapshot_end.restype = ip
     L.gpe_imag_time_iters.argtypes = [vp, C.c_long]
     L.gpe_imag_time_iters.restype = ip
     L.gpe_imag_time_until.argtypes = [vp, C.c_long, C.c_long, C.c_double, C.POINTER(C.c_long), dp]
@@ -254,6 +259,17 @@ class Device:
         e = GpeEnergies()
         self._check(self.L.gpe_calc_energy(self.h, C.byref(e)))
         return e.as_dict()
+
+    def snapshot_begin(self):
+        """Enqueue a copy of the state and its energies (gpe_snapshot_begin)."""
+        self._check(self.L.gpe_snapshot_begin(self.h))
+
+    def snapshot_end(self):
+        """(psi, energies) of the snapshot in flight (gpe_snapshot_end)."""
+        psi = np.empty(self.shape, dtype=np.complex128)
+        e = GpeEnergies()
+        self._check(self.L.gpe_snapshot_end(self.h, _dp(psi.view(np.float64)), C.byref(e)))
+        return psi, e.as_dict()
 
     def energy_noalloc(self, out_dev: int):
         """gpe_calc_energy_noalloc: the eight energies to device memory at

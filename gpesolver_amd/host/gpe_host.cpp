@@ -576,50 +576,80 @@ void AbstractGrossPitaevskiSolver::real_time_iters(long first_iter, long n) {
     }
 }
 
+// A snapshot begun at a checkpoint, finished once the next chunk of steps is
+// launched: its copy runs beside that chunk.
+void AbstractGrossPitaevskiSolver::finish_snapshot() {
+    snapshot_end();
+    p_mediator->save_checkpoint(cpsi);
+    enes.push_back(ene);
+}
+
 // Loop of solver.cpp:48-70: iterations 0..iter_imag inclusive, checkpoint +
-// energy every 100.  Iterations between two checkpoints run as one fused call.
+// energy every 100.  Iterations between two checkpoints run as one fused call;
+// the checkpoint's copy and energies run beside the next call (synchronously
+// when the convergence stop needs the energy at once).
 void AbstractGrossPitaevskiSolver::calc_initial_state() {
     long iter = 0;
     double last = 0.0;
-    bool have_last = false;
+    bool have_last = false, pending = false;
     it_done = 0;
     while (iter <= params->iter_imag) {
         const long next_cp = (iter % 100 == 0) ? iter : (iter / 100 + 1) * 100;
         const long upto = std::min(next_cp, params->iter_imag);
         imag_time_iters(upto - iter + 1);
+        if (pending) {
+            finish_snapshot();
+            pending = false;
+        }
         it_done += upto - iter + 1;
         iter = upto;
         if (iter % 100 == 0) {
-            export_data();
-            p_mediator->save_checkpoint(cpsi);
-            calc_energy();
-            const double e = ene.e_total / params->n_atoms;
-            if (it_tolerance > 0 && have_last && std::abs(e - last) < it_tolerance * std::abs(e)) break;
-            last = e;
-            have_last = true;
+            if (it_tolerance <= 0 && snapshot_begin()) {
+                pending = true;
+            } else {
+                export_data();
+                p_mediator->save_checkpoint(cpsi);
+                calc_energy();
+                const double e = ene.e_total / params->n_atoms;
+                if (it_tolerance > 0 && have_last && std::abs(e - last) < it_tolerance * std::abs(e)) break;
+                last = e;
+                have_last = true;
+            }
         }
         ++iter;
     }
+    if (pending) finish_snapshot();
     // As in the reference, cpsi keeps the state of the last checkpoint.
     p_mediator->save_energies(enes);
 }
 
 // Loop of solver.cpp:72-97: iterations 0..iter_real inclusive, update_edd
-// before each step, checkpoint + energy every 1000.
+// before each step, checkpoint + energy every 1000 (their copy runs beside the
+// next 1000 steps).
 void AbstractGrossPitaevskiSolver::calc_evolution() {
     long iter = 0;
+    bool pending = false;
     while (iter <= params->iter_real) {
         const long next_cp = (iter % 1000 == 0) ? iter : (iter / 1000 + 1) * 1000;
         const long upto = std::min(next_cp, params->iter_real);
         real_time_iters(iter, upto - iter + 1);
+        if (pending) {
+            finish_snapshot();
+            pending = false;
+        }
         iter = upto;
         if (iter % 1000 == 0) {
-            export_data();
-            p_mediator->save_checkpoint(cpsi);
-            calc_energy();
+            if (snapshot_begin()) {
+                pending = true;
+            } else {
+                export_data();
+                p_mediator->save_checkpoint(cpsi);
+                calc_energy();
+            }
         }
         ++iter;
     }
+    if (pending) finish_snapshot();
     p_mediator->save_data(cpsi);
     p_mediator->save_energies(enes);
 }
@@ -733,6 +763,25 @@ void HipGrossPitaevskiSolver::real_time_iters(long first_iter, long n) {
         gm[t] = params->gamma;
     }
     check(gpe_real_time_iters(h, n, g.data(), gm.data()));
+}
+
+bool HipGrossPitaevskiSolver::snapshot_begin() {
+    check(gpe_set_coupling(h, params->ggp11, params->gamma));
+    check(gpe_snapshot_begin(h));
+    return true;
+}
+
+void HipGrossPitaevskiSolver::snapshot_end() {
+    gpe_energies e;
+    check(gpe_snapshot_end(h, reinterpret_cast<double*>(cpsi.data()), &e));
+    ene.e_kin = e.e_kin;
+    ene.e_pot = e.e_pot;
+    ene.e_int = e.e_int;
+    ene.e_ext = e.e_ext;
+    ene.e_bmf = e.e_bmf;
+    ene.mu = e.mu;
+    ene.residual = e.residual;
+    ene.sum();
 }
 
 void HipGrossPitaevskiSolver::calc_energy() {

@@ -196,6 +196,13 @@ protected:
     // n consecutive iterations; subclasses may fuse them.
     virtual void imag_time_iters(long n);
     virtual void real_time_iters(long first_iter, long n);
+    // Output cadence without stopping the loop: snapshot_begin enqueues a
+    // copy of the state and its energies behind the launched steps (false:
+    // not supported, the loop then exports and computes them in place);
+    // snapshot_end makes them cpsi and ene.
+    virtual bool snapshot_begin() { return false; }
+    virtual void snapshot_end() {}
+    void finish_snapshot();
 
     virtual void calc_energy() = 0;
     virtual void init_containers() = 0;
@@ -242,6 +249,8 @@ protected:
     void export_data() override;
     void import_data() override;
     void import_pote() override;
+    bool snapshot_begin() override;
+    void snapshot_end() override;
 };
 
 // ---------------------------------------------------------------- manager

@@ -122,6 +122,17 @@ int gpe_imag_iter_step(gpe_handle *h);              /* imag_iter_linear_step + i
 int gpe_real_potential_half_step(gpe_handle *h);    /* cpu_solver.cpp:99-131 */
 int gpe_real_kinetic_step(gpe_handle *h);           /* fftw_split_solver.cpp:44-86 */
 int gpe_calc_energy(gpe_handle *h, gpe_energies *out); /* cpu_solver.cpp:137-194 (+ mu, residual) */
+/* The output cadence of the solver loops without stopping them: the reference
+ * copies the state and computes the energies synchronously every 100 IT /
+ * 1000 RT steps (export_data + save_checkpoint + calc_energy,
+ * src/solver/solver.cpp:54-59,81-86).  gpe_snapshot_begin enqueues, behind
+ * the work already launched, a device copy of the state and its energies
+ * (gpe_calc_energy's values), and their copy to pinned host memory on a second
+ * stream; later launches are not held back.  gpe_snapshot_end waits for that
+ * copy only and returns the state (as gpe_export_psi; psi may be NULL) and the
+ * energies (e may be NULL).  One snapshot per handle in flight. */
+int gpe_snapshot_begin(gpe_handle *h);
+int gpe_snapshot_end(gpe_handle *h, double *psi, gpe_energies *e);
 /* No counterpart in the reference.  For callers that cannot allocate (code
  * under hipGraph capture, tight serving loops): gpe_calc_energy with no
  * allocation, host copy or synchronisation.  The eight values are written,
