@@ -195,8 +195,11 @@ struct Dft<T, 32, SIGN, EX> {
     __device__ __forceinline__ static void run(cplx<T>* v) { dft_ab<T, 4, 8, SIGN, EX>(v); }
 };
 
+// 64 = 8 x 8 rather than 16 x 4: on a tile of 64-point lines both passes then
+// run twice the butterflies (threads) of a radix-16 pass.
 constexpr int pick_radix(int L) {
-    return (L % 16 == 0) ? 16
+    return (L == 64) ? 8
+           : (L % 16 == 0) ? 16
            : (L % 12 == 0) ? 12
            : (L % 8 == 0)  ? 8
            : (L % 6 == 0)  ? 6

@@ -23,7 +23,9 @@
 //            potential phase, starts the next Poisson solve (z R2C, y FFT of
 //            N|psi|^2) and runs the kinetic propagation along z and y.
 //   k_xpass  one block per tile of x-lines: Poisson x FFT, x Vdd(k), inverse
-//            x FFT on S1, and the kinetic propagation along x on Tk.
+//            x FFT on S1, and the kinetic propagation along x on Tk (the
+//            register x pass of 128 x 128 x 32 and 256 x 64 x 16 grids also
+//            runs the z propagation there, see kz_in_x).
 // One IT or RT step is one k_xpass plus one k_plane launch.  The kinetic
 // propagator exp(-i dt k^2/2m) = prod_ax exp(-i dt k_ax^2/2m) is separable, so
 // the 3D split step is three 1D (FFT, phase, inverse FFT) passes; the Poisson
@@ -120,6 +122,7 @@ struct PlaneArgs {
     cplx<T>* tko;       // written by M_KIN (== tk unless the buffers ping-pong)
     cplx<T>* s1o;       // written by M_RHO_FWD (== s1 unless the buffers ping-pong)
     int split;          // 1: two blocks per plane (grid 2 nx), see k_plane
+    int kz_skip;        // M_KIN leaves the z propagation to the x pass (LineArgs::kinz)
     const T* V;
     T* phi;
     const double* part_in;
@@ -169,6 +172,8 @@ struct LineArgs {
     const double* ky2;  // padded lattice ky^2, [2ny]
     const double* kz2;  // padded lattice kz^2, [nz+1]
     LinePlan px2, px;
+    const cplx<T>* ez;  // (cos fz(k) - 1, sin fz(k)) of the z kinetic phase fz
+    int kinz;           // kinetic tiles also run the z propagation (one tile = one y row, all z)
 };
 
 // |c|^2 in FP64 from a value of precision T: exact products of FP32
@@ -1010,9 +1015,9 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         if (mode & M_OUT) {
             for (int p = threadIdx.x; p < npl; p += NT) a.out[plane_off + p] = Q[(p / nz) * qs + p % nz];
         }
-        if (mode & M_KIN) {
+        if ((mode & M_KIN) && !a.kz_skip) {
             if (mode & M_OUT) __syncthreads();  // the psi output above has read Q
-            // z propagation: rows j, length nz
+            // z propagation: rows j, length nz (unless the x pass runs it)
             if constexpr (Pol::kRegZ) {
                 // tables named through the LDS base, so their reads compile to LDS loads
                 const cplx<T>* TWs = P + max(ny2 * nzh, ny * zs);
@@ -1662,6 +1667,7 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
         } else {
             TB[L + m] = a.ex[m];
             TB[2 * L + m] = a.twlx[m];
+            if (m < nz) TB[3 * L + m] = a.ez[m];
         }
     }
     __syncthreads();
@@ -1678,6 +1684,17 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = v[k1];
     __syncthreads();
+    if constexpr (!POISSON && (LPB == 32 || LPB == 16)) {
+        // the z propagation of the kinetic step: with LPB = nz the tile holds
+        // whole z rows, row k1 B + b of E being element k1 B + b of every z
+        // line of one y row after the first x stage; exp(-i dt kz^2/2m)
+        // commutes with the x transform, so it applies to these rows as to
+        // psi itself
+        if (a.kinz) {
+            kin_z_reg<T, LPB, 256>(E, L, LS, TB + 3 * L);
+            __syncthreads();
+        }
+    }
     // forward stage 2, multiply, inverse stage 1 -- rows k1 = b + B r
     cplx<T> w[R][B];
 #pragma unroll
@@ -1739,7 +1756,7 @@ struct XReg {
 };
 
 // LDS: exchange area (XReg::EXA complex) + tables (Poisson 2 * 2NX,
-// kinetic 3 * NX complex).
+// kinetic 3 * NX + nz complex).
 // The kinetic tiles (twice the loads of a zero-padded Poisson tile) take the
 // first block indices so that they are dispatched first.
 template <typename T, int NX>
@@ -2650,6 +2667,22 @@ static int setup_tables(gpe_handle* h) {
     return 0;
 }
 
+// The z propagation of the kinetic step runs in the register x pass when its
+// kinetic tiles hold whole z rows (LPB_K = nz: nx = 128 with nz = 32, nx = 256
+// with nz = 16) and the plane pass is a compiled single-block shape; the
+// plane pass then leaves it out.  Plane and x pass arguments both read this,
+// so the pair of launches of one step always agrees.
+static bool kz_in_x(const gpe_handle* h) {
+    if (h->force_generic || !h->line_reg || h->pencil) return false;
+    const int nx = h->p.nx, ny = h->p.ny, nz = h->p.nz;
+    if (!((nx == 128 && nz == 32) || (nx == 256 && nz == 16))) return false;
+#define GPE_PL(NY, NZ) \
+    if (ny == NY && nz == NZ) return true;
+    GPE_PLANE_SHAPES(GPE_PL)
+#undef GPE_PL
+    return false;
+}
+
 template <typename T>
 static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     PlaneArgs<T> a;
@@ -2665,6 +2698,7 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.tk = a.tko = (cplx<T>*)h->tk;
     a.s1 = a.s1o = (cplx<T>*)h->s1;
     a.split = 0;
+    a.kz_skip = kz_in_x(h) ? 1 : 0;
     a.V = (const T*)h->V;
     a.phi = (T*)h->phi;
     a.tw = (const cplx<T>*)h->tw;
@@ -2721,6 +2755,8 @@ static LineArgs<T> line_args(gpe_handle* h, bool poisson, bool kinetic) {
     a.kz2 = h->kz2;
     a.px2 = h->px2;
     a.px = h->px;
+    a.ez = (const cplx<T>*)h->ez;
+    a.kinz = kz_in_x(h) ? 1 : 0;
     (void)kinetic;
     return a;
 }

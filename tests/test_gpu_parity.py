@@ -321,12 +321,15 @@ def test_generic_runtime_size_kernels(oracle, shape, force, monkeypatch):
         assert rel(dev.export_psi(), orc.psi) < (TOL_PSI[prec] if prec == hip.GPE_FP64 else 3e-6)
 
 
-@pytest.mark.parametrize("shape", [(128, 16, 8), (64, 12, 6), (256, 8, 6)])
+@pytest.mark.parametrize("shape", [(128, 16, 8), (64, 12, 6), (256, 8, 6), (256, 64, 16)])
 @pytest.mark.parametrize("line_lds", ["0", "1"])
 def test_x_pass_register_and_lds_kernels(oracle, shape, line_lds, monkeypatch):
     """nx = 64, 128, 256 run the register-resident x pass by default (the
     512-point Poisson lines of nx = 256 as 32 x 16) and the LDS Stockham x
-    pass with GPE_LINE_LDS=1; both against the oracle."""
+    pass with GPE_LINE_LDS=1; both against the oracle.  On 256x64x16 the
+    register x pass also runs the kinetic z propagation (its kinetic tiles
+    hold whole z rows) and the plane pass leaves it out; with GPE_LINE_LDS=1
+    the plane pass runs it."""
     monkeypatch.setenv("GPE_LINE_LDS", line_lds)
     for prec in (hip.GPE_FP64, hip.GPE_FP32):
         p, dev, orc = make_pair(oracle, shape, prec, pot="HARMONIC", init="GAUSS", n_atoms=500.0)
