@@ -502,6 +502,93 @@ __device__ __forceinline__ void kin_y_reg_store(cplx<T>* Q, int qs, const cplx<T
     }
 }
 
+// y propagation on twice the threads of kin_y_reg_store: A = 8 values per
+// thread, B = NY / 8 threads per column.  Thread (b, c) holds rows
+// j = B a + b: DFT_8 over a, twiddle W_NY^(b k1), LDS exchange.  The DFT_B
+// over b of row k1 then runs on a lane pair (lane bit h, holding
+// b = 2 m + h): DFT_B/2 over m, twiddle W_B^(h k), and the radix-2 stage
+// across the pair through DPP, so lane h holds X[k1 + 8 (k + h B/2)]; the y
+// phase ey, then the same steps backwards and the result straight from
+// registers to Tk[j][i][c].  The exchange area aliases Q.  Every thread of
+// the block reaches the barriers.
+template <typename T, int NY, int NZ, int NT>
+__device__ __forceinline__ void kin_y_pair_store(cplx<T>* Q, int qs, const cplx<T>* ey, const cplx<T>* twy,
+                                                 const cplx<T>* twyl, cplx<T>* tk_plane, int nx) {
+    constexpr int A = 8, B = NY / A, H = B / 2, R = A / H, NTH = B * NZ;
+    static_assert(NY == A * B && B % 2 == 0 && A % H == 0 && NTH <= NT && NTH % 64 == 0,
+                  "y length of the paired register pass");
+    constexpr int ES = NZ + 1;  // exchange row stride (odd: distinct banks)
+    const bool act = threadIdx.x < NTH;  // whole waves: the DPP stage needs both lanes of a pair
+    const int c = threadIdx.x % NZ, b = threadIdx.x / NZ;
+    cplx<T> v[A];
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < A; ++q) v[q] = Q[(B * q + b) * qs + c];
+        gpe::ct::Dft<T, A, -1, true>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < A; ++k1) v[k1] = tw_psi<T, -1>(v[k1], twy, twyl, (b * k1) % NY);
+    }
+    __syncthreads();  // Q is read: the exchange area may overwrite it
+    cplx<T>* E = Q;
+    if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * ES + c] = v[k1];
+    }
+    __syncthreads();
+    // lane pair (h) of column c2 takes rows k1 = g + H r
+    const int h = threadIdx.x & 1, u = threadIdx.x >> 1;
+    const int c2 = u % NZ, g = u / NZ;
+    cplx<T> w[R][H];
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k1 = g + H * r;
+#pragma unroll
+            for (int m = 0; m < H; ++m) w[r][m] = E[(k1 * B + 2 * m + h) * ES + c2];
+            gpe::ct::Dft<T, H, -1, true>::run(w[r]);
+            if (h) {
+#pragma unroll
+                for (int k = 1; k < H; ++k) w[r][k] = gpe::ct::mul_w<T, B, -1, true>(w[r][k], k);
+            }
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                const cplx<T> o = shfl_xor1(w[r][k]);
+                w[r][k] = h ? gpe::csub(o, w[r][k]) : gpe::cadd(w[r][k], o);
+                w[r][k] = gpe::kin_mul(w[r][k], ey[k1 + A * (k + H * h)], 1.0 / NY);
+            }
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                const cplx<T> o = shfl_xor1(w[r][k]);
+                w[r][k] = h ? gpe::csub(o, w[r][k]) : gpe::cadd(w[r][k], o);
+            }
+            if (h) {
+#pragma unroll
+                for (int k = 1; k < H; ++k) w[r][k] = gpe::ct::mul_w<T, B, +1, true>(w[r][k], k);
+            }
+            gpe::ct::Dft<T, H, +1, true>::run(w[r]);
+        }
+    }
+    __syncthreads();  // every read of E is done before it is rewritten
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k1 = g + H * r;
+#pragma unroll
+            for (int m = 0; m < H; ++m) E[(k1 * B + 2 * m + h) * ES + c2] = w[r][m];
+        }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * ES + c];
+#pragma unroll
+        for (int k1 = 1; k1 < A; ++k1) v[k1] = tw_psi<T, +1>(v[k1], twy, twyl, (b * k1) % NY);
+        gpe::ct::Dft<T, A, +1, true>::run(v);
+#pragma unroll
+        for (int q = 0; q < A; ++q) store_nt(&tk_plane[(size_t)(B * q + b) * nx * NZ + c], v[q]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The 2NY-point y transforms of the Poisson stages in registers (compiled
 // shapes with NY = 64 or 128): four-step with A = 16 values per thread and
@@ -1004,9 +1091,11 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                             c = gpe::phase_rot(c, -a.hdt * veff);
                         }
                         Q[qi] = c;
-                        nrm = c.x * c.x + c.y * c.y;
                     }
                 }
+                // the density of the Poisson start: |psi|^2 before the phase,
+                // which leaves it unchanged (the split pass's Poisson-start
+                // block never applies the phase, and both agree bit for bit)
                 rho_r[q] = nrm;
                 nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
             }
@@ -1037,8 +1126,15 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
             // y propagation in registers, result straight to Tk
             if constexpr (Pol::kRegY) {
                 const cplx<T>* TWs = P + max(ny2 * nzh, ny * zs);
-                kin_y_reg_store<T, Pol::kNY, Pol::kNZ, NT>(Q, qs, TWs + a.tw_n + nz, TWs + a.py.tw,
-                                                           TWs + a.tw_n + nz + ny, a.tko + (size_t)i * nz, nx);
+                // the paired pass where its thread count fits the block
+                constexpr int BP = Pol::kNY / 8;
+                constexpr bool kPair = BP * Pol::kNZ <= NT && (BP * Pol::kNZ) % 64 == 0 && 8 % (BP / 2) == 0;
+                if constexpr (kPair)
+                    kin_y_pair_store<T, Pol::kNY, Pol::kNZ, NT>(Q, qs, TWs + a.tw_n + nz, TWs + a.py.tw,
+                                                                TWs + a.tw_n + nz + ny, a.tko + (size_t)i * nz, nx);
+                else
+                    kin_y_reg_store<T, Pol::kNY, Pol::kNZ, NT>(Q, qs, TWs + a.tw_n + nz, TWs + a.py.tw,
+                                                               TWs + a.tw_n + nz + ny, a.tko + (size_t)i * nz, nx);
             }
         } else if (mode & M_KIN) {
             // y propagation: columns k, length ny
@@ -1578,13 +1674,15 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
     }
 }
 
-// One block per x-plane, or (split) two: block i and block i + nx both
-// finish the Poisson solve and apply the phase (real time) or the stencil
-// (imaginary time) of plane i; the first then runs the kinetic z / y
-// propagation (and any psi, fi3d or norm output), the second starts the next
-// Poisson solve.  The two halves of the plane
-// work run on different CUs, at the price of reading the plane twice; the
-// outputs go to the other buffer of a ping-pong pair (s1o, tko) so no
+// One block per x-plane, or (split) two.  Block i finishes the Poisson solve
+// and applies the phase (real time) or the stencil (imaginary time) of plane
+// i, then runs the kinetic z / y propagation (and any psi, fi3d or norm
+// output); block i + nx starts the next Poisson solve.  In imaginary time it
+// also finishes the Poisson solve and runs the stencil, since the density is
+// that of the updated psi; in real time the phase leaves |psi|^2 unchanged,
+// so it starts from |psi|^2 of the plane as read.  The two halves of the
+// plane work run on different CUs, at the price of reading the plane twice;
+// the outputs go to the other buffer of a ping-pong pair (s1o, tko) so no
 // block overwrites an input another block may not have read yet.
 template <typename T, int NT, class Pol>
 __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
@@ -1592,10 +1690,12 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
     __shared__ double red[NT / 64 + 1];
     int i = blockIdx.x, mode = a.mode;
     if (a.split) {
-        if (i < a.nx)
+        if (i < a.nx) {
             mode &= ~M_RHO_FWD;
-        else
+        } else {
             mode = (mode & ~(M_KIN | M_OUT | M_PHI_STORE | M_NORM_OUT)) | M_NO_PSI_OUT;
+            if (mode & (M_PHASE_FULL | M_PHASE_HALF)) mode &= ~(M_PINV | M_PHASE_FULL | M_PHASE_HALF);
+        }
         i = i < a.nx ? i : i - a.nx;
     }
     plane_body<T, NT, Pol>(a, i, mode, lds_raw, red);
