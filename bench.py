@@ -40,7 +40,12 @@ Prints ONE JSON line.  Besides steps/s it reports:
                 (the recipe of gpesolver_amd.warmstart_experiment, fewer
                 epochs), then batched IT to |dE/N| < 1e-6 on held-out
                 instances from the FNO's prediction against plain
-                MULTIPLE_GAUSS: wall times (inference included) and steps.
+                MULTIPLE_GAUSS: wall times (inference included) and steps;
+  it_batch_sweep
+                the batched IT sweep (64 instances over 64x64x16, 96x96x24 and
+                128x128x32, each to |dE/N| < 1e-6 or 5000 steps) in one batch
+                with the batch autotuner's x-pass variants, against the same
+                instances one after another on autotuned single handles.
 """
 from __future__ import annotations
 
@@ -219,6 +224,26 @@ def host_cores() -> dict:
     return {"cpu_count": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
+def batch_sweep(n: int) -> dict:
+    """The batched IT sweep: n instances over 64x64x16, 96x96x24 and
+    128x128x32, FP32, each to |dE/N| < 1e-6 between 100-step checks or 5000
+    steps, in one gpe_batch with the batch autotuner's x-pass variants,
+    against the same instances one after another on autotuned single handles."""
+    from gpesolver_amd import sweep
+
+    setups = sweep.instances(n)
+    rb = sweep.run_batch(setups, 5000, 100, 1e-6, tune_steps=50)
+    rs = sweep.run_sequential(setups, 5000, 100, 1e-6, tune_steps=200)
+    de = np.abs(rb["e_per_atom"] - rs["e_per_atom"]) / np.abs(rs["e_per_atom"])
+    return {"instances": n, "grids": [list(s) for s in sweep.SHAPES], "tol_dE_per_N": 1e-6, "max_steps": 5000,
+            "batch_s": rb["wall_s"], "batch_tune_s": rb["tune_s"], "sequential_s": rs["wall_s"],
+            "speedup": rs["wall_s"] / rb["wall_s"], "batch_converged": rb["converged"],
+            "sequential_converged": rs["converged"], "batch_G_point_steps_per_s": rb["G_point_steps_per_s"],
+            "sequential_G_point_steps_per_s": rs["G_point_steps_per_s"],
+            "steps_mean": float(np.mean(rb["steps"])), "energy_rel_diff_max": float(de.max()),
+            "batch_tuned_per_nx": rb["tuned_per_nx"]}
+
+
 def warm_vs_plain(n_train: int, n_test: int, epochs: int, seed: int = 20261203) -> dict:
     """Train the warm-start FNO in this run and time batched IT, warm vs plain."""
     from gpesolver_amd import fno as F
@@ -265,6 +290,8 @@ def main():
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg (trains the FNO, ~2 min)")
     ap.add_argument("--warm-train", type=int, default=2000, help="warm-start leg: generated training states")
     ap.add_argument("--warm-epochs", type=int, default=40, help="warm-start leg: training epochs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched IT sweep leg (~20 s)")
+    ap.add_argument("--batch-n", type=int, default=64, help="batched IT sweep leg: instances")
     ap.add_argument("--rt-only", action="store_true",
                     help="profiling runs: the RT leg only, on the default launch shapes (split plane pass, "
                          "register x pass: what the autotuner picks at this grid), no autotune and no IT legs, "
@@ -363,6 +390,8 @@ def main():
 
     if not args.no_warm:
         extra["it_warm_vs_plain"] = warm_vs_plain(args.warm_train, 64, args.warm_epochs)
+    if not args.no_batch:
+        extra["it_batch_sweep"] = batch_sweep(args.batch_n)
 
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib << 20)

@@ -33,6 +33,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1288,11 +1289,13 @@ struct PencilPolGeneric {
 
 template <int NY, int NZ, int CC, int RR>
 struct PencilPolCT {
+    // CC / RR bound the tiles (the line bounds of the FFTs below); the tiles
+    // themselves come at run time (gpe_set_tiles / the autotuner may lower them)
     static constexpr int kC = CC, kR = RR;
     template <class A>
-    __device__ static constexpr int C(const A&) { return CC; }
+    __device__ static int C(const A& a) { return a.pen_c; }
     template <class A>
-    __device__ static constexpr int R(const A&) { return RR; }
+    __device__ static int R(const A& a) { return a.pen_r; }
     template <typename T, int NT, int S>
     __device__ static void y2(cplx<T>* b, int nl, int es, int ls, const PlaneArgs<T>& a) {
         gpe::ct::fft<T, 2 * NY, NT, CC, S>(b, nl, es, ls, a.tw + a.py2.tw);
@@ -2503,11 +2506,13 @@ struct gpe_handle {
     void* pen_h = nullptr;
     double* pen_part = nullptr;  // norm partial per (plane, row tile)
     int pen_c = 0, pen_r = 0;   // column / row tile shapes
+    int pen_c_max = 0, pen_r_max = 0;  // the bounds the pencil kernels were built for
     bool pen_yreg = true;       // register y passes of the pencil pass (GPE_PENCIL_YREG=0: LDS column tiles)
     size_t pen_lds = 0;
     LinePlan px, py, pz, px2, py2, pz2;
-    // launch shapes
+    // launch shapes: kz / k columns per LDS x-pass tile, and their bounds
     int cp = 0, ck = 0;
+    int cp_max = 0, ck_max = 0;
     size_t plane_lds = 0, line_lds = 0;
     size_t plane_stage = 0;  // extra LDS of the compiled plane shapes (staged tables)
     int tw_n = 0;            // entries of the twiddle table
@@ -4103,10 +4108,10 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
     if (rc) return bad(h->err);
     // launch shapes: Poisson tiles of cp kz-columns, kinetic tiles of ck k
     if (2L * p->nx > kLineMaxE) return bad("x length too large for the line pass");
-    h->cp = std::min(p->nz, std::max(1, kLineMaxE / (2 * p->nx)));
-    h->cp = std::min(h->cp, 12);
-    h->ck = std::min(p->nz, std::max(1, kLineMaxE / p->nx));
-    h->ck = std::min(h->ck, 16);
+    h->cp_max = std::min(p->nz, std::max(1, kLineMaxE / (2 * p->nx)));
+    h->ck_max = std::min(p->nz, std::max(1, kLineMaxE / p->nx));
+    h->cp = std::min(h->cp_max, 12);
+    h->ck = std::min(h->ck_max, 16);
     h->plane_lds = (size_t)std::max(2L * p->ny * (p->nz + 1), (long)p->ny * (2 * p->nz + 1)) * cs;
     h->plane_stage = (size_t)(h->tw_n + 2 * (p->ny + p->nz)) * cs;
     h->line_lds = (size_t)std::max(2L * p->nx * (h->cp + 1), (long)p->nx * (h->ck + 1)) * cs;
@@ -4133,7 +4138,9 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
 #undef GPE_PP
         }
         h->pen_lds = (size_t)std::max(2L * p->ny * (h->pen_c + 1), (long)h->pen_r * (2 * p->nz + 1)) * cs;
-        const size_t nrt = (p->ny + h->pen_r - 1) / h->pen_r;
+        h->pen_c_max = h->pen_c;
+        h->pen_r_max = h->pen_r;
+        const size_t nrt = p->ny;  // partials for row tiles down to one row (gpe_set_tiles)
         if (hipMalloc(&h->pen_gw, n * cs) || hipMalloc(&h->pen_h, n * cs) ||
             hipMalloc(&h->pen_part, (size_t)p->nx * nrt * sizeof(double)))
             return bad("device allocation failed");
@@ -4446,9 +4453,40 @@ static void drop_graphs(gpe_handle* h) {
     }
 }
 
+// Launch tiles within their bounds (values <= 0, or pen_yreg < 0, keep the
+// current one); the LDS sizes follow.
+static int apply_tiles(gpe_handle* h, int cp, int ck, int pen_c, int pen_r, int pen_yreg) {
+    if (cp > h->cp_max || ck > h->ck_max) return fail(h, "x-pass tile beyond the built bound");
+    if (h->pencil && (pen_c > h->pen_c_max || pen_r > h->pen_r_max))
+        return fail(h, "pencil tile beyond the built bound");
+    if (DISPATCH(h, materialize, h)) return 1;
+    const gpe_params& p = h->p;
+    const size_t cs = csize(h);
+    if (cp > 0) h->cp = cp;
+    if (ck > 0) h->ck = ck;
+    h->line_lds = (size_t)std::max(2L * p.nx * (h->cp + 1), (long)p.nx * (h->ck + 1)) * cs;
+    if (h->pencil) {
+        if (pen_c > 0) h->pen_c = pen_c;
+        if (pen_r > 0) h->pen_r = pen_r;
+        if (pen_yreg >= 0) h->pen_yreg = pen_yreg != 0;
+        h->pen_lds = (size_t)std::max(2L * p.ny * (h->pen_c + 1), (long)h->pen_r * (2 * p.nz + 1)) * cs;
+    }
+    drop_graphs(h);
+    return 0;
+}
+
+// Tile candidates: the bound and the usual power-of-two steps below it.
+static std::vector<int> tile_candidates(int bound, int first) {
+    std::vector<int> c{first};
+    for (int v : {bound, 16, 12, 8, 4, 2})
+        if (v >= 1 && v <= bound && std::find(c.begin(), c.end(), v) == c.end()) c.push_back(v);
+    return c;
+}
+
 struct TuneResult {
     bool split, line_reg, graphs;
     double rt_us, it_us;
+    int cp, ck, pen_c, pen_r, pen_yreg;
 };
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int, int>, TuneResult> g_tune;
@@ -4516,11 +4554,47 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
                     rc = tune_time(s, steps, &rt, &itu);
                     const double cost = rt + itu;
                     if (!rc && (!have || cost < best_cost)) {
-                        best = TuneResult{sp != 0, lr != 0, gr != 0, rt, itu};
+                        best = TuneResult{sp != 0, lr != 0, gr != 0, rt, itu, s->cp, s->ck, s->pen_c, s->pen_r,
+                                          s->pen_yreg ? 1 : 0};
                         best_cost = cost;
                         have = true;
                     }
                 }
+        // then the tiles of the chosen variant, one parameter at a time: the
+        // LDS x pass's kz / k columns per block, the pencil pass's register y
+        // passes, row tile and (LDS y passes) column tile
+        auto try_tiles = [&](int cp, int ck, int pc, int pr, int yr) -> int {
+            if (apply_tiles(s, cp, ck, pc, pr, yr)) return 1;
+            double rt = 0, itu = 0;
+            if (tune_time(s, steps, &rt, &itu)) return 1;
+            if (rt + itu < best_cost) {
+                best = TuneResult{best.split, best.line_reg, best.graphs, rt, itu, s->cp, s->ck, s->pen_c, s->pen_r,
+                                  s->pen_yreg ? 1 : 0};
+                best_cost = rt + itu;
+            }
+            return apply_tiles(s, best.cp, best.ck, best.pen_c, best.pen_r, best.pen_yreg);
+        };
+        if (!rc) {
+            rc = DISPATCH(s, materialize, s);
+            s->plane_split = best.split;
+            s->line_reg = best.line_reg;
+            s->use_graphs = best.graphs;
+            drop_graphs(s);
+        }
+        if (!rc && !best.line_reg) {
+            for (int v : tile_candidates(s->cp_max, s->cp))
+                if (!rc && v != best.cp) rc = try_tiles(v, 0, 0, 0, -1);
+            for (int v : tile_candidates(s->ck_max, s->ck))
+                if (!rc && v != best.ck) rc = try_tiles(0, v, 0, 0, -1);
+        }
+        if (!rc && s->pencil) {
+            if (h->p.ny == 256 && !s->force_generic) rc = try_tiles(0, 0, 0, 0, best.pen_yreg ? 0 : 1);
+            for (int v : tile_candidates(s->pen_r_max, s->pen_r))
+                if (!rc && v != best.pen_r) rc = try_tiles(0, 0, 0, v, -1);
+            if (!best.pen_yreg || h->p.ny != 256)
+                for (int v : tile_candidates(s->pen_c_max, s->pen_c))
+                    if (!rc && v != best.pen_c) rc = try_tiles(0, 0, v, 0, -1);
+        }
         if (rc) {
             const std::string m = s->err;
             gpe_destroy(s);
@@ -4535,12 +4609,148 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
     if (h->plane_split != best.split || h->line_reg != best.line_reg) drop_graphs(h);
     h->plane_split = best.split;
     h->line_reg = best.line_reg;
+    if (apply_tiles(h, best.cp, best.ck, h->pencil ? best.pen_c : 0, h->pencil ? best.pen_r : 0,
+                    h->pencil ? best.pen_yreg : -1))
+        return 1;
     const char* ng = getenv("GPE_NO_GRAPH");
     h->use_graphs = best.graphs && !(ng && ng[0] == '1');
     if (config) *config = (best.split ? 1 : 0) | (best.line_reg ? 2 : 0) | (h->use_graphs ? 4 : 0);
     if (rt_step_us) *rt_step_us = best.rt_us;
     if (it_step_us) *it_step_us = best.it_us;
     return 0;
+}
+
+int gpe_set_tiles(gpe_handle* h, int cp, int ck, int pen_c, int pen_r, int pen_yreg) {
+    if (!h) return 1;
+    return apply_tiles(h, cp, ck, pen_c, pen_r, pen_yreg);
+}
+
+int gpe_get_tiles(const gpe_handle* h, int* cur, int* bound) {
+    if (!h) return 1;
+    if (cur) {
+        cur[0] = h->cp;
+        cur[1] = h->ck;
+        cur[2] = h->pencil ? h->pen_c : 0;
+        cur[3] = h->pencil ? h->pen_r : 0;
+        cur[4] = h->pencil ? (h->pen_yreg ? 1 : 0) : 0;
+    }
+    if (bound) {
+        bound[0] = h->cp_max;
+        bound[1] = h->ck_max;
+        bound[2] = h->pencil ? h->pen_c_max : 0;
+        bound[3] = h->pencil ? h->pen_r_max : 0;
+        bound[4] = h->pencil ? 1 : 0;
+    }
+    return 0;
+}
+
+// Batch autotuner: per x length (one x-pass launch per length in the batch),
+// the register x pass where it exists and the LDS x pass at each kz-column
+// tile, timed as whole batched IT steps on a scratch batch of the same grids
+// (uniform fields), one length at a time with the others at their current or
+// already chosen variant.  The plane pass has one form in the batch.
+int gpe_batch_autotune(gpe_batch* b, long steps, int* config) {
+    if (!b) return fail(nullptr, "null batch");
+    if (steps < 1) return bfail(b, "autotune needs steps >= 1");
+    const int n = (int)b->inst.size();
+    std::vector<gpe_params> ps;
+    for (gpe_handle* h : b->inst) ps.push_back(h->p);
+    gpe_batch* s = nullptr;
+    if (gpe_batch_create(ps.data(), n, b->prec, &s)) return bfail(b, "autotune scratch batch: " + g_create_err);
+    int rc = 0;
+    for (int q = 0; q < n && !rc; ++q) {
+        gpe_handle* hs = s->inst[q];
+        hs->line_reg = b->inst[q]->line_reg;
+        std::vector<double> psi(2 * hs->N, 1.0);
+        rc = gpe_load_buffer(hs, psi.data()) || apply_tiles(hs, b->inst[q]->cp, b->inst[q]->ck, 0, 0, -1);
+        if (rc) (void)bfail(b, hs->err);
+    }
+    if (!rc && hipMalloc((void**)&s->d_esum, (size_t)n * 16 * sizeof(double)) != hipSuccess)
+        rc = bfail(b, "autotune scratch batch: device allocation failed");
+    auto time_batch = [&](double* us) -> int {
+        for (gpe_handle* h : s->inst) {
+            if (DISPATCH(h, materialize, h)) return 1;
+            if (h->cur != 0) {
+                BCHECK(hipMemcpyAsync(h->store[0], h->store[1], h->N * csize(h), hipMemcpyDeviceToDevice, s->stream));
+                h->cur = 0;
+            }
+            h->pcur = 0;
+        }
+        s->cur = s->pcur = 0;
+        s->active.clear();
+        for (int q = 0; q < n; ++q) s->active.push_back(q);
+        const int r = s->prec == GPE_FP64 ? batch_setup<double>(s) : batch_setup<float>(s);
+        if (r) return bfail(b, s->err);
+        auto run = [&](long k) { return s->prec == GPE_FP64 ? batch_it<double>(s, k) : batch_it<float>(s, k); };
+        if (run(kGraphSteps + 2)) return bfail(b, s->err);  // warm-up, captures the graphs
+        BCHECK(hipStreamSynchronize(s->stream));
+        hipEvent_t e0, e1;
+        BCHECK(hipEventCreate(&e0));
+        BCHECK(hipEventCreate(&e1));
+        (void)hipEventRecord(e0, s->stream);
+        const int rr = run(steps);
+        (void)hipEventRecord(e1, s->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rr) return bfail(b, s->err);
+        *us = 1e3 * ms / steps;
+        return 0;
+    };
+    std::vector<int> lengths;
+    for (gpe_handle* h : s->inst)
+        if (std::find(lengths.begin(), lengths.end(), h->p.nx) == lengths.end()) lengths.push_back(h->p.nx);
+    double best_us = 0;
+    if (!rc) rc = time_batch(&best_us);
+    for (int nx : lengths) {
+        if (rc) break;
+        gpe_handle* h0 = nullptr;
+        for (gpe_handle* h : s->inst)
+            if (h->p.nx == nx) {
+                h0 = h;
+                break;
+            }
+        const bool reg_ok = !h0->force_generic && (nx == 128 || nx == 64 || nx == 256);
+        bool best_reg = h0->line_reg && reg_ok;
+        int best_cp = h0->cp;
+        std::vector<std::pair<bool, int>> cands;
+        if (reg_ok) cands.push_back({true, best_cp});
+        for (int v : tile_candidates(h0->cp_max, h0->cp)) cands.push_back({false, v});
+        for (const auto& c : cands) {
+            if (rc) break;
+            if (c.first == best_reg && (c.first || c.second == best_cp)) continue;  // timed already
+            for (gpe_handle* h : s->inst)
+                if (h->p.nx == nx) {
+                    h->line_reg = c.first;
+                    if (apply_tiles(h, c.second, 0, 0, 0, -1)) rc = bfail(b, h->err);
+                }
+            double us = 0;
+            if (!rc) rc = time_batch(&us);
+            if (!rc && us < best_us) {
+                best_us = us;
+                best_reg = c.first;
+                best_cp = c.second;
+            }
+        }
+        for (gpe_handle* h : s->inst)
+            if (h->p.nx == nx) {
+                h->line_reg = best_reg;
+                if (!rc && apply_tiles(h, best_cp, 0, 0, 0, -1)) rc = bfail(b, h->err);
+            }
+    }
+    if (!rc) {
+        for (int q = 0; q < n && !rc; ++q) {
+            gpe_handle* h = b->inst[q];
+            h->line_reg = s->inst[q]->line_reg;
+            if (apply_tiles(h, s->inst[q]->cp, 0, 0, 0, -1)) rc = bfail(b, h->err);
+            if (config) config[q] = (h->line_reg ? 2 : 0) | (h->cp << 8);
+        }
+        batch_drop_graphs(b);
+    }
+    gpe_batch_destroy(s);
+    return rc;
 }
 
 int gpe_set_timing(gpe_handle* h, int on) {

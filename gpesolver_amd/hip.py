@@ -24,9 +24,9 @@ EXPORTS = (
     "gpe_real_potential_half_step", "gpe_real_kinetic_step", "gpe_calc_energy", "gpe_calc_energy_noalloc",
     "gpe_snapshot_begin", "gpe_snapshot_end",
     "gpe_imag_time_iters", "gpe_real_time_iters", "gpe_imag_time_until", "gpe_synchronize", "gpe_set_timing",
-    "gpe_get_timing", "gpe_autotune",
+    "gpe_get_timing", "gpe_autotune", "gpe_set_tiles", "gpe_get_tiles",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_instance", "gpe_batch_last_error", "gpe_batch_it_until",
-    "gpe_batch_set_fallback", "gpe_batch_fallback_taken",
+    "gpe_batch_set_fallback", "gpe_batch_fallback_taken", "gpe_batch_autotune",
     "gpe_fno_create", "gpe_fno_destroy", "gpe_fno_last_error", "gpe_fno_num_params", "gpe_fno_get_params",
     "gpe_fno_set_params", "gpe_fno_get_grads", "gpe_fno_forward", "gpe_fno_loss_grad", "gpe_fno_adam_step",
     "gpe_gemm_test",
@@ -119,6 +119,12 @@ def load(path: str = LIB_PATH):
     L.gpe_real_time_iters.restype = ip
     L.gpe_autotune.argtypes = [vp, C.c_long, C.POINTER(C.c_int), dp, dp]
     L.gpe_autotune.restype = ip
+    L.gpe_set_tiles.argtypes = [vp, ip, ip, ip, ip, ip]
+    L.gpe_set_tiles.restype = ip
+    L.gpe_get_tiles.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.gpe_get_tiles.restype = ip
+    L.gpe_batch_autotune.argtypes = [vp, C.c_long, C.POINTER(C.c_int)]
+    L.gpe_batch_autotune.restype = ip
     L.gpe_set_timing.argtypes = [vp, ip]
     L.gpe_set_timing.restype = ip
     L.gpe_get_timing.argtypes = [vp, dp, dp, dp]
@@ -305,7 +311,20 @@ class Device:
         rt, it = C.c_double(), C.c_double()
         self._check(self.L.gpe_autotune(self.h, steps, C.byref(cfg), C.byref(rt), C.byref(it)))
         return {"split_plane_pass": bool(cfg.value & 1), "register_x_pass": bool(cfg.value & 2),
-                "graph_replay": bool(cfg.value & 4), "rt_step_us": rt.value, "it_step_us": it.value}
+                "graph_replay": bool(cfg.value & 4), "rt_step_us": rt.value, "it_step_us": it.value,
+                "tiles": self.tiles()["cur"]}
+
+    TILE_KEYS = ("cp", "ck", "pen_c", "pen_r", "pen_yreg")
+
+    def tiles(self) -> dict:
+        """Launch tiles and their bounds (gpe_get_tiles)."""
+        cur, bound = (C.c_int * 5)(), (C.c_int * 5)()
+        self._check(self.L.gpe_get_tiles(self.h, cur, bound))
+        return {"cur": dict(zip(self.TILE_KEYS, cur[:])), "bound": dict(zip(self.TILE_KEYS, bound[:]))}
+
+    def set_tiles(self, cp: int = 0, ck: int = 0, pen_c: int = 0, pen_r: int = 0, pen_yreg: int = -1):
+        """Force launch tiles (gpe_set_tiles; 0, or pen_yreg -1, keeps one)."""
+        self._check(self.L.gpe_set_tiles(self.h, cp, ck, pen_c, pen_r, pen_yreg))
 
     def set_timing(self, on: bool):
         self._check(self.L.gpe_set_timing(self.h, 1 if on else 0))
@@ -375,6 +394,15 @@ class Batch:
             rc = self.L.gpe_batch_set_fallback(self.b, i, _dp(a.view(np.float64)))
         if rc != 0:
             raise GpeError(self.L.gpe_batch_last_error(self.b).decode())
+
+    def autotune(self, steps: int = 100) -> list:
+        """Per instance: register or LDS x pass and its cp tile, chosen per x
+        length by timing whole batched steps (gpe_batch_autotune)."""
+        n = len(self.instances)
+        cfg = (C.c_int * n)()
+        if self.L.gpe_batch_autotune(self.b, steps, cfg) != 0:
+            raise GpeError(self.L.gpe_batch_last_error(self.b).decode())
+        return [{"register_x_pass": bool(c & 2), "cp": c >> 8} for c in cfg[:]]
 
     def fallbacks_taken(self) -> np.ndarray:
         return np.array([bool(self.L.gpe_batch_fallback_taken(self.b, i)) for i in range(len(self.instances))])

@@ -229,6 +229,30 @@ int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float
  * bits from an untuned one (defaults: split, register x pass). */
 int gpe_autotune(gpe_handle *h, long steps, int *config, double *rt_step_us, double *it_step_us);
 
+/* Launch tiles (no reference counterpart): kz columns per Poisson tile (cp)
+ * and k columns per kinetic tile (ck) of the LDS x pass, and the pencil plane
+ * pass's column tile (pen_c, LDS y passes), row tile (pen_r) and register y
+ * passes (pen_yreg, 0/1).  gpe_autotune searches them after the launch
+ * variant (cp / ck when it picks the LDS x pass, the pencil ones on pencil
+ * grids); gpe_set_tiles forces them (a value <= 0, or pen_yreg < 0, keeps the
+ * current one) within the bounds the kernels were built for, which
+ * gpe_get_tiles reports: cur[5] and bound[5] in the order cp, ck, pen_c,
+ * pen_r, pen_yreg (pencil entries 0 on single-block grids).  Tiles change
+ * only how lines are grouped into blocks: the LDS x pass gives the same
+ * values at every cp / ck, the pencil pass the same up to the summation
+ * order of its FP64 norm partials. */
+int gpe_set_tiles(gpe_handle *h, int cp, int ck, int pen_c, int pen_r, int pen_yreg);
+int gpe_get_tiles(const gpe_handle *h, int *cur, int *bound);
+
+/* Batch autotuner (no reference counterpart): per x length of the batch's
+ * grids (one x-pass launch per length), the register x pass where it exists
+ * and the LDS x pass at each cp tile are timed as whole batched IT steps
+ * (`steps` steps each) on a scratch batch of the same grids and precision,
+ * one length at a time, and the fastest is kept for every instance of that
+ * length.  config (length n, may be NULL) receives per instance bit 1 =
+ * register x pass and bits 8.. = cp. */
+int gpe_batch_autotune(gpe_batch *b, long steps, int *config);
+
 /* Wait for all work of h. */
 int gpe_synchronize(gpe_handle *h);
 

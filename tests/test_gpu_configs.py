@@ -130,54 +130,30 @@ def test_first_steps_against_oracle_128x128x32(oracle):
         dev.close()
 
 
-def sweep_instance(rng, shape):
-    """One instance of the batched IT sweep: edd ~ U[1.2, 1.6], n_atoms
-    log-uniform in [5e3, 5e4], omega_z ~ U[100, 140] Hz; MEXICAN with
-    dd = 1500 nm and MULTIPLE_GAUSS(2) (4-cell widths) plus 1e-3 rad noise."""
-    nx, ny, nz = shape
-    edd = rng.uniform(1.2, 1.6)
-    n_atoms = 10 ** rng.uniform(np.log10(5e3), np.log10(5e4))
-    wz = rng.uniform(100.0, 140.0)
-    p = P.PhysicalParameters.from_physical(nx, ny, nz, *D, n_atoms=n_atoms, edd=edd, omega_hz=(60.0, 60.0, wz),
-                                           dd_nm=1500.0)
-    V = P.potential("MEXICAN", p)
-    psi = P.add_phase_noise(P.wavefunction("MULTIPLE_GAUSS", p, n_gauss_max=2, sigma_cells=4.0), 1e-3,
-                            int(rng.integers(1 << 30)))
-    return p, V, psi
-
-
 def test_config3_batched_it_sweep():
     """64 instances over 64x64x16, 96x96x24 and 128x128x32 (evenly mixed), FP32
-    bulk, one launch per pass over every active instance, each to
-    |dE/N| < 1e-6 between 100-step checks or 5000 steps."""
-    import time
+    bulk, one launch per pass and grid shape over every active instance, x-pass
+    variants from the batch autotuner, each to |dE/N| < 1e-6 between 100-step
+    checks or 5000 steps; one instance per grid again on its own handle."""
+    from gpesolver_amd import sweep
 
-    rng = np.random.default_rng(20261203)
-    shapes = [(64, 64, 16), (96, 96, 24), (128, 128, 32)]
-    setups = [sweep_instance(rng, shapes[q % 3]) for q in range(64)]
-    b = hip.Batch([s[0].to_gpe() for s in setups], hip.GPE_FP32)
-    for d, (p, V, psi) in zip(b.instances, setups):
-        d.load_pote(V)
-        d.load_buffer(psi)
-    t0 = time.perf_counter()
-    steps, e = b.it_until(5000, check_every=100, tol=1e-6)
-    wall = time.perf_counter() - t0
-    conv = steps < 5000
-    pts = np.array([s[0].nx * s[0].ny * s[0].nz for s in setups], dtype=np.float64)
-    print(f"\nconfig 3 sweep: {wall:.2f} s, {conv.sum()} of 64 converged, steps mean {steps.mean():.0f} "
-          f"max {steps.max()}, {np.sum(steps * pts) / wall / 1e9:.2f} G point-steps/s", flush=True)
+    setups = sweep.instances(64)
+    r = sweep.run_batch(setups, 5000, 100, 1e-6, tune_steps=30)
+    steps, e = r["steps"], r["e_per_atom"]
+    print(f"\nconfig 3 sweep: {r['wall_s']:.2f} s (tuning {r['tune_s']:.2f} s), {r['converged']} of 64 converged, "
+          f"steps mean {steps.mean():.0f} max {steps.max()}, {r['G_point_steps_per_s']:.2f} G point-steps/s, "
+          f"tuned {r['tuned_per_nx']}", flush=True)
     assert np.all(np.isfinite(e)) and np.all(steps % 100 == 0) and np.all(steps <= 5000)
-    assert conv.sum() >= 32
-    # one instance of each grid again on its own handle: the same stop and E/N
+    assert r["converged"] >= 32
+    assert set(r["tuned_per_nx"]) == {"64", "96", "128"}
     for q in range(3):
         p, V, psi = setups[q]
         d = hip.Device(p.to_gpe(), hip.GPE_FP32)
         d.load_pote(V)
         d.load_buffer(psi)
         s1, e1 = d.it_until(5000, check_every=100, tol=1e-6)
-        print(f"  instance {q} {shapes[q]}: batch {int(steps[q])} steps E/N {e[q]:.9e}, single {s1} steps "
+        print(f"  instance {q} {sweep.SHAPES[q]}: batch {int(steps[q])} steps E/N {e[q]:.9e}, single {s1} steps "
               f"E/N {e1:.9e}", flush=True)
         assert abs(s1 - int(steps[q])) <= 200
         assert abs(e1 - e[q]) <= 1e-5 * abs(e1)
         d.close()
-    b.close()
