@@ -1560,6 +1560,268 @@ __global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
     }
 }
 
+// The row-tile step of the pencil pass with its z transforms in registers
+// (nz = 64, 32 rows y0 .. y0 + 31 of plane i per 256-thread block).  Thread
+// (r, b), b < 8, works on row y0 + r:
+//   Poisson finish: the 128-point inverse of the Hermitian-extended G row as
+//     a four-step 16 x 8 (x[8 a + b] loaded straight from G, DFT_16,
+//     twiddle, one LDS exchange, DFT_8 of rows k1 = b, b + 8), which leaves
+//     the thread holding z = k1 + 16 k2 = 8 a' + b, a' = k1 / 8 + 2 k2 < 8,
+//     of the real output;
+//   the stencil or the phase at those eight points (psi and V loaded at the
+//     kernel start);
+//   Poisson start: the forward 128-point transform of the zero-padded
+//     density, whose four-step input is exactly those eight points (x[8 a' +
+//     b], a' < 8), one exchange, H from registers;
+//   kinetic z: the 64-point four-step 8 x 8 of the same eight points (two
+//     exchanges), W from registers.
+// Four LDS exchanges in all, against eight Stockham passes of the LDS tile.
+constexpr int kPzRows = 32;
+template <typename T>
+struct PzReg {
+    static constexpr int NZ = 64, L = 128, B = 8, LS = kPzRows + 1;
+    static constexpr int EXA = L * LS;                      // exchange area (complex)
+    static constexpr int TAB = L + 3 * NZ;                  // W_128, W_64, lo parts of W_64, ez
+    static constexpr size_t lds_bytes() { return (size_t)(EXA + TAB) * sizeof(cplx<T>); }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pencil_rz(PlaneArgs<T> a) {
+    using Z = PzReg<T>;
+    constexpr int NZ = Z::NZ, L = Z::L, B = Z::B, LS = Z::LS;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[256 / 64 + 1];
+    cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
+    cplx<T>* TW2 = E + Z::EXA;  // W_128^m
+    cplx<T>* TW1 = TW2 + L;     // W_64^m
+    cplx<T>* TWL = TW1 + NZ;    // lo parts of W_64^m
+    cplx<T>* EZ = TWL + NZ;     // z kinetic phase
+    const int mode = a.mode;
+    const int nx = a.nx, ny = a.ny, nz = NZ;
+    const int npl = ny * nz;
+    const int nrt = ny / kPzRows;
+    const int i = blockIdx.x / nrt, tr = blockIdx.x - i * nrt, y0 = tr * kPzRows;
+    const int b = threadIdx.x & 7, r = threadIdx.x >> 3;
+    const int j = y0 + r;
+    const size_t plane_off = (size_t)i * npl;
+    const size_t row_off = plane_off + (size_t)j * nz;
+    const bool is_it = (mode & M_IT) != 0;
+    const bool have_psi = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL | M_KIN | M_OUT | M_RHO_FWD)) != 0;
+    const bool need_v = (mode & (M_IT | M_PHASE_HALF | M_PHASE_FULL)) != 0;
+    cplx<T>* gw = a.pen_gw + plane_off;  // [y][c]
+    cplx<T>* hs = a.pen_h + plane_off;   // [y][kz - 1]
+    // the thread's points z = 8 q + b, q < 8
+    // ---- loads first: G row (with its Hermitian mirror), psi, V ----------
+    cplx<T> v[16];
+    if (mode & M_PINV) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int kz = 8 * q + b;  // 0 .. 127
+            cplx<T> g = gpe::mk(T(0), T(0));
+            if (kz >= 1 && kz <= NZ) {
+                g = gw[(size_t)j * nz + kz - 1];
+            } else if (kz > NZ) {
+                const cplx<T> m = gw[(size_t)j * nz + (L - kz) - 1];
+                g = gpe::mk(m.x, -m.y);
+            }
+            v[q] = g;
+        }
+    }
+    cplx<T> c[8];
+    T vv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int z = 8 * q + b;
+        c[q] = gpe::mk(T(0), T(0));
+        vv[q] = T(0);
+        if (have_psi) {
+            if (!is_it && (mode & M_IN_TK))
+                c[q] = a.tk[((size_t)j * nx + i) * nz + z];
+            else
+                c[q] = a.in[row_off + z];
+        }
+        if (need_v) vv[q] = a.V[row_off + z];
+    }
+    for (int m = threadIdx.x; m < L; m += 256) TW2[m] = a.tw[a.pz2.tw + m];
+    if (threadIdx.x < NZ) {
+        TW1[threadIdx.x] = a.tw[a.pz.tw + threadIdx.x];
+        TWL[threadIdx.x] = a.twlz[threadIdx.x];
+        EZ[threadIdx.x] = a.ez[threadIdx.x];
+    }
+    double pg, pgm;
+    pencil_couplings(a, mode, &pg, &pgm);
+    double s_in = 1.0;
+    if (mode & M_NORM_IN) s_in = 1.0 / sqrt(sum_partials(a.part_in, nx, red));
+    __syncthreads();  // tables published
+    // ---- Poisson finish: Phi at z = 8 q + b ---------------------------------
+    T phi[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) phi[q] = T(0);
+    if (mode & M_PINV) {
+        gpe::ct::Dft<T, 16, +1>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) v[k1] = gpe::cmulc(v[k1], TW2[(b * k1) % L]);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) E[(k1 * B + b) * LS + r] = v[k1];
+        __syncthreads();
+        const T sc = (T)(a.pinv_scale * s_in * s_in);
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) {
+            const int k1 = b + B * rr;
+            cplx<T> w[B];
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) w[bb] = E[(k1 * B + bb) * LS + r];
+            gpe::ct::Dft<T, B, +1>::run(w);
+            // output z = k1 + 16 k2 = 8 (rr + 2 k2) + b, k2 < 4 (z < 64)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) phi[rr + 2 * k2] = w[k2].x * sc;
+        }
+        if (mode & M_PHI_STORE) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) a.phi[row_off + 8 * q + b] = phi[q];
+        }
+        __syncthreads();  // the exchange area is read before it is reused
+    } else if (mode & M_PHI_LOAD) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) phi[q] = a.phi[row_off + 8 * q + b];
+    }
+    // ---- psi update ---------------------------------------------------------
+    const bool iint = (i >= 1 && i <= nx - 2);
+    const bool jint = (j >= 1 && j <= ny - 2);
+    double nacc = 0.0;
+    T rho[8];
+    if (is_it) {
+        const cplx<T>* in = a.in;
+        const T sv = (T)s_in;
+        const T dtau = (T)a.dtau, cx = (T)a.cx, cy = (T)a.cy, cz = (T)a.cz, gq = (T)a.gq, gmq = (T)a.gmq,
+                cdd = (T)a.cdd;
+        auto ld = [&](size_t e) {
+            const cplx<T> t = in[e];
+            return gpe::mk((T)(t.x * s_in), (T)(t.y * s_in));
+        };
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = 8 * q + b;
+            const size_t e = row_off + k;
+            cplx<T> o;
+            if (iint && jint && k >= 1 && k <= nz - 2) {
+                const cplx<T> cc = gpe::mk((T)(c[q].x * s_in), (T)(c[q].y * s_in));
+                const cplx<T> xm = in[e - npl], xp = in[e + npl];
+                const cplx<T> ym = ld(e - nz), yp = ld(e + nz), zm = ld(e - 1), zp = ld(e + 1);
+                const T vt = vv[q] + cdd * phi[q];
+                cplx<T> c1;
+                c1.x = cx * (xm.x * sv + xp.x * sv - T(2) * cc.x) + cy * (ym.x + yp.x - T(2) * cc.x) +
+                       cz * (zm.x + zp.x - T(2) * cc.x) + cc.x * vt;
+                c1.y = cx * (xm.y * sv + xp.y * sv - T(2) * cc.y) + cy * (ym.y + yp.y - T(2) * cc.y) +
+                       cz * (zm.y + zp.y - T(2) * cc.y) + cc.y * vt;
+                const T nrm = cc.x * cc.x + cc.y * cc.y;
+                const T ab = sqrt(nrm);
+                const T f = gq * nrm + gmq * (ab * ab * ab);
+                o.x = cc.x - dtau * c1.x;
+                o.y = cc.y - dtau * c1.y;
+                o.x = o.x - dtau * (f * cc.x);
+                o.y = o.y - dtau * (f * cc.y);
+            } else {
+                o = a.psii[e];
+            }
+            a.out[e] = o;
+            c[q] = o;
+            rho[q] = o.x * o.x + o.y * o.y;
+            nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+        }
+    } else {
+        const float tv = (float)(a.hdt * a.cv), tphi = (float)(a.hdt * a.cv * a.cdd), tg = (float)(a.hdt * pg),
+                    tgm = (float)(a.hdt * pgm);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = 8 * q + b;
+            cplx<T> cc = c[q];
+            if (!(mode & M_IN_TK) && (mode & M_NORM_IN)) cc = gpe::mk((T)(cc.x * s_in), (T)(cc.y * s_in));
+            const T nrm = cc.x * cc.x + cc.y * cc.y;
+            if ((mode & (M_PHASE_HALF | M_PHASE_FULL)) && iint && jint && k >= 1 && k <= nz - 2) {
+                bool done = false;
+                if constexpr (sizeof(T) == 4) {
+                    const float t32 = tv * vv[q] + tphi * phi[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
+                    if (fabsf(t32) <= 0.0625f) {
+                        cc = gpe::phase_rot_small(cc, -t32);
+                        done = true;
+                    }
+                }
+                if (!done) {
+                    const double dn = dens64(cc);
+                    const double veff =
+                        a.cv * ((double)vv[q] + a.cdd * (double)phi[q]) + pg * dn + pgm * (dn * sqrt(dn));
+                    cc = gpe::phase_rot(cc, -a.hdt * veff);
+                }
+            }
+            c[q] = cc;
+            rho[q] = nrm;  // the density before the phase, which leaves it unchanged
+            nacc += (double)cc.x * (double)cc.x + (double)cc.y * (double)cc.y;
+            if (mode & M_OUT) a.out[row_off + k] = cc;
+        }
+    }
+    // ---- Poisson start: H = forward transform of the zero-padded density ---
+    if (mode & M_RHO_FWD) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = gpe::mk(q < 8 ? rho[q] : T(0), T(0));
+        gpe::ct::Dft<T, 16, -1>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) v[k1] = gpe::cmul(v[k1], TW2[(b * k1) % L]);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) E[(k1 * B + b) * LS + r] = v[k1];
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) {
+            const int k1 = b + B * rr;
+            cplx<T> w[B];
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) w[bb] = E[(k1 * B + bb) * LS + r];
+            gpe::ct::Dft<T, B, -1>::run(w);
+#pragma unroll
+            for (int k2 = 0; k2 < 5; ++k2) {
+                const int kz = k1 + 16 * k2;  // H holds kz = 1 .. nz
+                if (kz >= 1 && kz <= NZ) hs[(size_t)j * nz + kz - 1] = w[k2];
+            }
+        }
+        __syncthreads();
+    }
+    // ---- kinetic z: 64-point four-step 8 x 8 on z = 8 q + b -----------------
+    if (mode & M_KIN) {
+        cplx<T> u[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) u[q] = c[q];
+        gpe::ct::Dft<T, 8, -1, true>::run(u);
+#pragma unroll
+        for (int k1 = 1; k1 < 8; ++k1) u[k1] = tw_psi<T, -1>(u[k1], TW1, TWL, (b * k1) % NZ);
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) E[(k1 * B + b) * LS + r] = u[k1];
+        __syncthreads();
+        // row k1 = b: X[b + 8 k2]
+        cplx<T> w[8];
+#pragma unroll
+        for (int bb = 0; bb < 8; ++bb) w[bb] = E[(b * B + bb) * LS + r];
+        gpe::ct::Dft<T, 8, -1, true>::run(w);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) w[k2] = gpe::kin_mul(w[k2], EZ[b + 8 * k2], 1.0 / NZ);
+        gpe::ct::Dft<T, 8, +1, true>::run(w);
+#pragma unroll
+        for (int bb = 1; bb < 8; ++bb) w[bb] = tw_psi<T, +1>(w[bb], TW1, TWL, (bb * b) % NZ);
+        // row b's slots are read and rewritten by this thread only
+#pragma unroll
+        for (int bb = 0; bb < 8; ++bb) E[(b * B + bb) * LS + r] = w[bb];
+        __syncthreads();
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) u[k1] = E[(k1 * B + b) * LS + r];
+        gpe::ct::Dft<T, 8, +1, true>::run(u);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) gw[(size_t)j * nz + 8 * q + b] = u[q];
+    }
+    if (mode & M_NORM_OUT) {
+        const double sum = block_sum<256>(nacc, red);
+        if (threadIdx.x == 0) a.pen_part[(size_t)i * nrt + tr] = sum;
+    }
+}
+
 // blocks [0, nx*nct): y2 of H -> S1o (M_RHO_FWD); [nx*nct, 2 nx*nct): y
 // kinetic of W -> Tk (M_KIN); the first block of each plane also sums the
 // plane's tile norm partials in order (M_NORM_OUT)
@@ -2508,6 +2770,7 @@ struct gpe_handle {
     int pen_c = 0, pen_r = 0;   // column / row tile shapes
     int pen_c_max = 0, pen_r_max = 0;  // the bounds the pencil kernels were built for
     bool pen_yreg = true;       // register y passes of the pencil pass (GPE_PENCIL_YREG=0: LDS column tiles)
+    bool pen_zreg = true;       // register z passes of its row tiles (GPE_PENCIL_ZREG=0: LDS row tiles)
     size_t pen_lds = 0;
     LinePlan px, py, pz, px2, py2, pz2;
     // launch shapes: kz / k columns per LDS x-pass tile, and their bounds
@@ -2930,8 +3193,14 @@ static int launch_pencil(gpe_handle* h, const PlaneArgs<T>& a) {
                                   h->pen_lds, h->stream, e0, nullptr, 0, a);
         GPE_CHECK(hipGetLastError());
     }
-    hipExtLaunchKernelGGL((k_pencil_r<T, kPencilThreads, Pol>), dim3(nx * nrt), dim3(kPencilThreads), h->pen_lds,
-                          h->stream, ci ? nullptr : e0, cf ? nullptr : e1, 0, a);
+    // register z passes: compiled pencil shapes with nz = 64 and 32-row tiles
+    const bool zreg = kYReg && h->pen_zreg && nz == PzReg<T>::NZ && h->pen_r == kPzRows && ny % kPzRows == 0;
+    if (zreg)
+        hipExtLaunchKernelGGL((k_pencil_rz<T>), dim3(nx * nrt), dim3(256), PzReg<T>::lds_bytes(), h->stream,
+                              ci ? nullptr : e0, cf ? nullptr : e1, 0, a);
+    else
+        hipExtLaunchKernelGGL((k_pencil_r<T, kPencilThreads, Pol>), dim3(nx * nrt), dim3(kPencilThreads),
+                              h->pen_lds, h->stream, ci ? nullptr : e0, cf ? nullptr : e1, 0, a);
     GPE_CHECK(hipGetLastError());
     if (cf) {
         if (yreg)
@@ -4121,6 +4390,8 @@ int gpe_create(const gpe_params* p, int precision, gpe_handle** out) {
         // GPE_PENCIL_TILES="c,r" caps the tile shapes (tests: partial last tiles)
         const char* yr = getenv("GPE_PENCIL_YREG");
         h->pen_yreg = !(yr && yr[0] == '0');
+        const char* zr = getenv("GPE_PENCIL_ZREG");
+        h->pen_zreg = !(zr && zr[0] == '0');
         const char* pt = getenv("GPE_PENCIL_TILES");
         int tc = 0, tr = 0;
         if (pt && sscanf(pt, "%d,%d", &tc, &tr) == 2 && tc > 0 && tr > 0) {
@@ -4455,7 +4726,7 @@ static void drop_graphs(gpe_handle* h) {
 
 // Launch tiles within their bounds (values <= 0, or pen_yreg < 0, keep the
 // current one); the LDS sizes follow.
-static int apply_tiles(gpe_handle* h, int cp, int ck, int pen_c, int pen_r, int pen_yreg) {
+static int apply_tiles(gpe_handle* h, int cp, int ck, int pen_c, int pen_r, int pen_yreg, int pen_zreg = -1) {
     if (cp > h->cp_max || ck > h->ck_max) return fail(h, "x-pass tile beyond the built bound");
     if (h->pencil && (pen_c > h->pen_c_max || pen_r > h->pen_r_max))
         return fail(h, "pencil tile beyond the built bound");
@@ -4469,6 +4740,7 @@ static int apply_tiles(gpe_handle* h, int cp, int ck, int pen_c, int pen_r, int 
         if (pen_c > 0) h->pen_c = pen_c;
         if (pen_r > 0) h->pen_r = pen_r;
         if (pen_yreg >= 0) h->pen_yreg = pen_yreg != 0;
+        if (pen_zreg >= 0) h->pen_zreg = pen_zreg != 0;
         h->pen_lds = (size_t)std::max(2L * p.ny * (h->pen_c + 1), (long)h->pen_r * (2 * p.nz + 1)) * cs;
     }
     drop_graphs(h);
@@ -4486,7 +4758,7 @@ static std::vector<int> tile_candidates(int bound, int first) {
 struct TuneResult {
     bool split, line_reg, graphs;
     double rt_us, it_us;
-    int cp, ck, pen_c, pen_r, pen_yreg;
+    int cp, ck, pen_c, pen_r, pen_yreg, pen_zreg;
 };
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int, int>, TuneResult> g_tune;
@@ -4555,7 +4827,7 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
                     const double cost = rt + itu;
                     if (!rc && (!have || cost < best_cost)) {
                         best = TuneResult{sp != 0, lr != 0, gr != 0, rt, itu, s->cp, s->ck, s->pen_c, s->pen_r,
-                                          s->pen_yreg ? 1 : 0};
+                                          s->pen_yreg ? 1 : 0, s->pen_zreg ? 1 : 0};
                         best_cost = cost;
                         have = true;
                     }
@@ -4563,16 +4835,16 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
         // then the tiles of the chosen variant, one parameter at a time: the
         // LDS x pass's kz / k columns per block, the pencil pass's register y
         // passes, row tile and (LDS y passes) column tile
-        auto try_tiles = [&](int cp, int ck, int pc, int pr, int yr) -> int {
-            if (apply_tiles(s, cp, ck, pc, pr, yr)) return 1;
+        auto try_tiles = [&](int cp, int ck, int pc, int pr, int yr, int zr = -1) -> int {
+            if (apply_tiles(s, cp, ck, pc, pr, yr, zr)) return 1;
             double rt = 0, itu = 0;
             if (tune_time(s, steps, &rt, &itu)) return 1;
             if (rt + itu < best_cost) {
                 best = TuneResult{best.split, best.line_reg, best.graphs, rt, itu, s->cp, s->ck, s->pen_c, s->pen_r,
-                                  s->pen_yreg ? 1 : 0};
+                                  s->pen_yreg ? 1 : 0, s->pen_zreg ? 1 : 0};
                 best_cost = rt + itu;
             }
-            return apply_tiles(s, best.cp, best.ck, best.pen_c, best.pen_r, best.pen_yreg);
+            return apply_tiles(s, best.cp, best.ck, best.pen_c, best.pen_r, best.pen_yreg, best.pen_zreg);
         };
         if (!rc) {
             rc = DISPATCH(s, materialize, s);
@@ -4589,6 +4861,8 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
         }
         if (!rc && s->pencil) {
             if (h->p.ny == 256 && !s->force_generic) rc = try_tiles(0, 0, 0, 0, best.pen_yreg ? 0 : 1);
+            if (!rc && h->p.ny == 256 && h->p.nz == 64 && !s->force_generic)
+                rc = try_tiles(0, 0, 0, 0, -1, best.pen_zreg ? 0 : 1);
             for (int v : tile_candidates(s->pen_r_max, s->pen_r))
                 if (!rc && v != best.pen_r) rc = try_tiles(0, 0, 0, v, -1);
             if (!best.pen_yreg || h->p.ny != 256)
@@ -4610,7 +4884,7 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
     h->plane_split = best.split;
     h->line_reg = best.line_reg;
     if (apply_tiles(h, best.cp, best.ck, h->pencil ? best.pen_c : 0, h->pencil ? best.pen_r : 0,
-                    h->pencil ? best.pen_yreg : -1))
+                    h->pencil ? best.pen_yreg : -1, h->pencil ? best.pen_zreg : -1))
         return 1;
     const char* ng = getenv("GPE_NO_GRAPH");
     h->use_graphs = best.graphs && !(ng && ng[0] == '1');
@@ -4620,9 +4894,9 @@ int gpe_autotune(gpe_handle* h, long steps, int* config, double* rt_step_us, dou
     return 0;
 }
 
-int gpe_set_tiles(gpe_handle* h, int cp, int ck, int pen_c, int pen_r, int pen_yreg) {
+int gpe_set_tiles(gpe_handle* h, int cp, int ck, int pen_c, int pen_r, int pen_yreg, int pen_zreg) {
     if (!h) return 1;
-    return apply_tiles(h, cp, ck, pen_c, pen_r, pen_yreg);
+    return apply_tiles(h, cp, ck, pen_c, pen_r, pen_yreg, pen_zreg);
 }
 
 int gpe_get_tiles(const gpe_handle* h, int* cur, int* bound) {
@@ -4633,6 +4907,7 @@ int gpe_get_tiles(const gpe_handle* h, int* cur, int* bound) {
         cur[2] = h->pencil ? h->pen_c : 0;
         cur[3] = h->pencil ? h->pen_r : 0;
         cur[4] = h->pencil ? (h->pen_yreg ? 1 : 0) : 0;
+        cur[5] = h->pencil ? (h->pen_zreg ? 1 : 0) : 0;
     }
     if (bound) {
         bound[0] = h->cp_max;
@@ -4640,6 +4915,7 @@ int gpe_get_tiles(const gpe_handle* h, int* cur, int* bound) {
         bound[2] = h->pencil ? h->pen_c_max : 0;
         bound[3] = h->pencil ? h->pen_r_max : 0;
         bound[4] = h->pencil ? 1 : 0;
+        bound[5] = h->pencil ? 1 : 0;
     }
     return 0;
 }

@@ -119,7 +119,7 @@ def load(path: str = LIB_PATH):
     L.gpe_real_time_iters.restype = ip
     L.gpe_autotune.argtypes = [vp, C.c_long, C.POINTER(C.c_int), dp, dp]
     L.gpe_autotune.restype = ip
-    L.gpe_set_tiles.argtypes = [vp, ip, ip, ip, ip, ip]
+    L.gpe_set_tiles.argtypes = [vp, ip, ip, ip, ip, ip, ip]
     L.gpe_set_tiles.restype = ip
     L.gpe_get_tiles.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.gpe_get_tiles.restype = ip
@@ -314,17 +314,18 @@ class Device:
                 "graph_replay": bool(cfg.value & 4), "rt_step_us": rt.value, "it_step_us": it.value,
                 "tiles": self.tiles()["cur"]}
 
-    TILE_KEYS = ("cp", "ck", "pen_c", "pen_r", "pen_yreg")
+    TILE_KEYS = ("cp", "ck", "pen_c", "pen_r", "pen_yreg", "pen_zreg")
 
     def tiles(self) -> dict:
         """Launch tiles and their bounds (gpe_get_tiles)."""
-        cur, bound = (C.c_int * 5)(), (C.c_int * 5)()
+        cur, bound = (C.c_int * 6)(), (C.c_int * 6)()
         self._check(self.L.gpe_get_tiles(self.h, cur, bound))
         return {"cur": dict(zip(self.TILE_KEYS, cur[:])), "bound": dict(zip(self.TILE_KEYS, bound[:]))}
 
-    def set_tiles(self, cp: int = 0, ck: int = 0, pen_c: int = 0, pen_r: int = 0, pen_yreg: int = -1):
-        """Force launch tiles (gpe_set_tiles; 0, or pen_yreg -1, keeps one)."""
-        self._check(self.L.gpe_set_tiles(self.h, cp, ck, pen_c, pen_r, pen_yreg))
+    def set_tiles(self, cp: int = 0, ck: int = 0, pen_c: int = 0, pen_r: int = 0, pen_yreg: int = -1,
+                  pen_zreg: int = -1):
+        """Force launch tiles (gpe_set_tiles; 0, or a flag -1, keeps one)."""
+        self._check(self.L.gpe_set_tiles(self.h, cp, ck, pen_c, pen_r, pen_yreg, pen_zreg))
 
     def set_timing(self, on: bool):
         self._check(self.L.gpe_set_timing(self.h, 1 if on else 0))

@@ -231,17 +231,19 @@ int gpe_autotune(gpe_handle *h, long steps, int *config, double *rt_step_us, dou
 
 /* Launch tiles (no reference counterpart): kz columns per Poisson tile (cp)
  * and k columns per kinetic tile (ck) of the LDS x pass, and the pencil plane
- * pass's column tile (pen_c, LDS y passes), row tile (pen_r) and register y
- * passes (pen_yreg, 0/1).  gpe_autotune searches them after the launch
- * variant (cp / ck when it picks the LDS x pass, the pencil ones on pencil
- * grids); gpe_set_tiles forces them (a value <= 0, or pen_yreg < 0, keeps the
- * current one) within the bounds the kernels were built for, which
- * gpe_get_tiles reports: cur[5] and bound[5] in the order cp, ck, pen_c,
- * pen_r, pen_yreg (pencil entries 0 on single-block grids).  Tiles change
- * only how lines are grouped into blocks: the LDS x pass gives the same
- * values at every cp / ck, the pencil pass the same up to the summation
- * order of its FP64 norm partials. */
-int gpe_set_tiles(gpe_handle *h, int cp, int ck, int pen_c, int pen_r, int pen_yreg);
+ * pass's column tile (pen_c, LDS y passes), row tile (pen_r), register y
+ * passes (pen_yreg, 0/1) and register z passes of its row tiles (pen_zreg,
+ * 0/1; nz = 64 with 32-row tiles).  gpe_autotune searches them after the
+ * launch variant (cp / ck when it picks the LDS x pass, the pencil ones on
+ * pencil grids); gpe_set_tiles forces them (a value <= 0, or pen_yreg /
+ * pen_zreg < 0, keeps the current one) within the bounds the kernels were
+ * built for, which gpe_get_tiles reports: cur[6] and bound[6] in the order
+ * cp, ck, pen_c, pen_r, pen_yreg, pen_zreg (pencil entries 0 on single-block
+ * grids).  Tiles change only how lines are grouped into blocks: the LDS x
+ * pass gives the same values at every cp / ck, the pencil pass the same up
+ * to the summation order of its FP64 norm partials (and the rounding of the
+ * register z passes against the LDS ones). */
+int gpe_set_tiles(gpe_handle *h, int cp, int ck, int pen_c, int pen_r, int pen_yreg, int pen_zreg);
 int gpe_get_tiles(const gpe_handle *h, int *cur, int *bound);
 
 /* Batch autotuner (no reference counterpart): per x length of the batch's

@@ -133,22 +133,26 @@ def test_large_rt_first_steps_against_oracle(oracle, monkeypatch):
     orc.load_buffer(psi)
     orc.rt_steps(0, 3)
     g, gm = couplings(p, 0, 3)
-    # ny = 256 runs the register y passes by default; "0" keeps the LDS column tiles covered
-    for prec, yreg in ((hip.GPE_FP64, "1"), (hip.GPE_FP64, "0"), (hip.GPE_FP32, "1"), (hip.GPE_FP32, "0")):
+    # ny = 256 runs the register y passes and (nz = 64) the register z passes by default; "0" keeps
+    # the LDS column tiles and the LDS row tiles covered
+    for prec, yreg, zreg in [(pr, y, z) for pr in (hip.GPE_FP64, hip.GPE_FP32)
+                             for y, z in (("1", "1"), ("0", "1"), ("1", "0"))]:
         monkeypatch.setenv("GPE_PENCIL_YREG", yreg)
+        monkeypatch.setenv("GPE_PENCIL_ZREG", zreg)
         dev = dev_of(p, prec, V, psi)
         dev.rt_steps(3, g, gm)
         r = rel(dev.export_psi(), orc.psi)
-        print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} (GPE_PENCIL_YREG={yreg}) 3 RT steps vs "
-              f"oracle: psi {r:.3e}", flush=True)
+        print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} (GPE_PENCIL_YREG={yreg}, "
+              f"GPE_PENCIL_ZREG={zreg}) 3 RT steps vs oracle: psi {r:.3e}", flush=True)
         assert r < (1e-10 if prec == hip.GPE_FP64 else 8e-7)  # FP32: 3.4e-7 measured
         dev.close()
 
 
 def test_large_it_and_poisson_against_oracle(oracle, monkeypatch):
     """256x256x64: Poisson hook, 3 fused imaginary-time steps and the energies against the oracle,
-    FP64 and FP32, each with the register y passes (the default at ny = 256) and with the LDS
-    column tiles; the two column paths of each precision also against each other."""
+    FP64 and FP32, each with the register y and z passes (the default at ny = 256, nz = 64), with
+    the LDS column tiles and with the LDS row tiles; the paths of each precision also against each
+    other."""
     oracle.set_threads(min(16, os.cpu_count() or 1))
     p, op, V, psi = large_case(oracle)
     orc = oracle.Oracle(op)
@@ -159,26 +163,30 @@ def test_large_it_and_poisson_against_oracle(oracle, monkeypatch):
     orc.it_steps(3)
     eo = orc.energy()
     out = {}
-    for prec, yreg in ((hip.GPE_FP64, "1"), (hip.GPE_FP64, "0"), (hip.GPE_FP32, "1"), (hip.GPE_FP32, "0")):
+    for prec, yreg, zreg in [(pr, y, z) for pr in (hip.GPE_FP64, hip.GPE_FP32)
+                             for y, z in (("1", "1"), ("0", "1"), ("1", "0"))]:
         monkeypatch.setenv("GPE_PENCIL_YREG", yreg)
+        monkeypatch.setenv("GPE_PENCIL_ZREG", zreg)
         dev = dev_of(p, prec, V, psi)
         dev.calc_fi3d()
         rf = rel(dev.export_fi3d(), fi0)
         dev.it_steps(3)
-        out[(prec, yreg)] = dev.export_psi()
-        r = rel(out[(prec, yreg)], orc.psi)
+        out[(prec, yreg, zreg)] = dev.export_psi()
+        r = rel(out[(prec, yreg, zreg)], orc.psi)
         e = dev.energy()
         de = abs(e["e_total"] - eo["e_total"]) / abs(eo["e_total"])
-        print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} (GPE_PENCIL_YREG={yreg}) vs oracle: "
-              f"fi3d {rf:.3e}, psi after 3 IT steps {r:.3e}, E {de:.3e}", flush=True)
+        print(f"\n256x256x64 {'FP64' if prec == hip.GPE_FP64 else 'FP32'} (GPE_PENCIL_YREG={yreg}, "
+              f"GPE_PENCIL_ZREG={zreg}) vs oracle: fi3d {rf:.3e}, psi after 3 IT steps {r:.3e}, E {de:.3e}",
+              flush=True)
         fp64 = prec == hip.GPE_FP64
         # FP32: fi3d 1.45e-7, psi 5.9e-8, E 5.6e-8 measured
         assert rf < (1e-11 if fp64 else 3e-7)
         assert r < (1e-10 if fp64 else 1.5e-7)
         assert de < (1e-10 if fp64 else 1.2e-7)
         dev.close()
-    assert rel(out[(hip.GPE_FP64, "1")], out[(hip.GPE_FP64, "0")]) < 1e-13
-    assert rel(out[(hip.GPE_FP32, "1")], out[(hip.GPE_FP32, "0")]) < 1e-6
+    for other in (("0", "1"), ("1", "0")):
+        assert rel(out[(hip.GPE_FP64, "1", "1")], out[(hip.GPE_FP64, *other)]) < 1e-13
+        assert rel(out[(hip.GPE_FP32, "1", "1")], out[(hip.GPE_FP32, *other)]) < 1e-6
 
 
 def test_large_rt_500_steps_fp32_vs_fp64():
