@@ -1982,7 +1982,9 @@ __global__ void __launch_bounds__(NT) k_xpass(LineArgs<T> a) {
 // so a forward / multiply / inverse line costs two LDS exchanges.  Global
 // loads and stores go straight to / from registers; lines are the fastest
 // lane index, so neighbouring lanes touch neighbouring kz / k (contiguous).
-template <typename T, int L, int LPB, bool POISSON, int A = 16>
+// SPLIT: the two exchanges move the real parts, then the imaginary parts,
+// through one area of L (LPB + 1) reals (half the LDS of a complex area).
+template <typename T, int L, int LPB, bool POISSON, int A = 16, bool SPLIT = false>
 __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>* E, cplx<T>* TB) {
     constexpr int B = L / A, R = A / B;
     constexpr int LS = LPB + 1;  // padded row stride of the exchange area
@@ -2046,59 +2048,98 @@ __device__ __forceinline__ void xline_reg(const LineArgs<T>& a, int blk, cplx<T>
         else
             v[k1] = tw_psi<T, -1>(v[k1], tw, twl, (b * k1) % L);
     }
-#pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = v[k1];
-    __syncthreads();
-    if constexpr (!POISSON && (LPB == 32 || LPB == 16)) {
-        // the z propagation of the kinetic step: with LPB = nz the tile holds
-        // whole z rows, row k1 B + b of E being element k1 B + b of every z
-        // line of one y row after the first x stage; exp(-i dt kz^2/2m)
-        // commutes with the x transform, so it applies to these rows as to
-        // psi itself
-        if (a.kinz) {
-            kin_z_reg<T, LPB, 256>(E, L, LS, TB + 3 * L);
-            __syncthreads();
-        }
-    }
-    // forward stage 2, multiply, inverse stage 1 -- rows k1 = b + B r
-    cplx<T> w[R][B];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
+    // forward stage 2, multiply, inverse stage 1 of row k1 = b + B r (w[r] loaded)
+    auto stage2 = [&](cplx<T>(&wr)[B], int r) {
         const int k1 = b + B * r;
-#pragma unroll
-        for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * LS + ll];
-        gpe::ct::Dft<T, B, -1, !POISSON>::run(w[r]);
+        gpe::ct::Dft<T, B, -1, !POISSON>::run(wr);
         if (POISSON) {
 #pragma unroll
             for (int k2 = 0; k2 < B; ++k2) {
                 const T k2s = kx2s[k1 + A * k2] + ky2 + kz2;
                 const T vd = (k2s > T(1e-30)) ? kz2 * rcp_t(k2s) : T(0);
-                w[r][k2] = gpe::cscale(w[r][k2], vd);
+                wr[k2] = gpe::cscale(wr[k2], vd);
             }
         } else {
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) w[r][k2] = gpe::kin_mul(w[r][k2], exs[k1 + A * k2], 1.0 / L);
+            for (int k2 = 0; k2 < B; ++k2) wr[k2] = gpe::kin_mul(wr[k2], exs[k1 + A * k2], 1.0 / L);
         }
-        gpe::ct::Dft<T, B, +1, !POISSON>::run(w[r]);
+        gpe::ct::Dft<T, B, +1, !POISSON>::run(wr);
 #pragma unroll
         for (int bb = 1; bb < B; ++bb) {
             if constexpr (POISSON)
-                w[r][bb] = gpe::cmulc(w[r][bb], tw[(bb * k1) % L]);
+                wr[bb] = gpe::cmulc(wr[bb], tw[(bb * k1) % L]);
             else
-                w[r][bb] = tw_psi<T, +1>(w[r][bb], tw, twl, (bb * k1) % L);
+                wr[bb] = tw_psi<T, +1>(wr[bb], tw, twl, (bb * k1) % L);
         }
+    };
+    cplx<T> w[R][B];
+    if constexpr (SPLIT) {
+        static_assert(POISSON, "split exchange: Poisson tiles");
+        T* ER = reinterpret_cast<T*>(E);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h) __syncthreads();  // the real parts are read before the imaginary parts land
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) ER[(k1 * B + b) * LS + ll] = h ? v[k1].y : v[k1].x;
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int bb = 0; bb < B; ++bb) {
+                    const T e = ER[((b + B * r) * B + bb) * LS + ll];
+                    if (h) w[r][bb].y = e; else w[r][bb].x = e;
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) stage2(w[r], r);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            __syncthreads();  // every read of the area is done before it is rewritten
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int bb = 0; bb < B; ++bb) ER[((b + B * r) * B + bb) * LS + ll] = h ? w[r][bb].y : w[r][bb].x;
+            __syncthreads();
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) {
+                const T e = ER[(k1 * B + b) * LS + ll];
+                if (h) v[k1].y = e; else v[k1].x = e;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) E[(k1 * B + b) * LS + ll] = v[k1];
+        __syncthreads();
+        if constexpr (!POISSON && (LPB == 32 || LPB == 16)) {
+            // the z propagation of the kinetic step: with LPB = nz the tile holds
+            // whole z rows, row k1 B + b of E being element k1 B + b of every z
+            // line of one y row after the first x stage; exp(-i dt kz^2/2m)
+            // commutes with the x transform, so it applies to these rows as to
+            // psi itself
+            if (a.kinz) {
+                kin_z_reg<T, LPB, 256>(E, L, LS, TB + 3 * L);
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k1 = b + B * r;
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) w[r][bb] = E[(k1 * B + bb) * LS + ll];
+            stage2(w[r], r);
+        }
+        __syncthreads();  // every stage-2 read of E is done before it is rewritten
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k1 = b + B * r;
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) E[(k1 * B + bb) * LS + ll] = w[r][bb];
+        }
+        __syncthreads();
+        // inverse stage 2: thread b gathers Z[k1][b] for every k1
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * LS + ll];
     }
-    __syncthreads();  // every stage-2 read of E is done before it is rewritten
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int k1 = b + B * r;
-#pragma unroll
-        for (int bb = 0; bb < B; ++bb) E[(k1 * B + bb) * LS + ll] = w[r][bb];
-    }
-    __syncthreads();
-    // inverse stage 2: thread b gathers Z[k1][b] for every k1
-#pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) v[k1] = E[(k1 * B + b) * LS + ll];
     gpe::ct::Dft<T, A, +1, !POISSON>::run(v);
     if (valid) {
         // streaming stores: the next pass reads them from memory anyway, and
@@ -2116,11 +2157,22 @@ struct XReg {
     static constexpr int AP = LP / 16 > 16 ? 32 : 16, AK = 16;
     static constexpr int LPB_P = 256 / (LP / AP);         // lines per 256-thread block
     static constexpr int LPB_K = 256 / (LK / AK);
-    // exchange area: L (LPB + 1) complex
-    static constexpr int EXA = LP * (LPB_P + 1) > LK * (LPB_K + 1) ? LP * (LPB_P + 1) : LK * (LPB_K + 1);
+    // the 512-point Poisson lines exchange real and imaginary parts in turn,
+    // so their area is no larger than the kinetic tiles' (3 blocks per CU in
+    // FP32 instead of 2; in FP64 the registers allow one, and the complex
+    // exchange with fewer barriers is faster)
+    static constexpr bool SPLIT_P = NX == 256;
+    template <typename T>
+    static constexpr bool split_p() { return SPLIT_P && sizeof(T) == 4; }
+    // exchange area (complex units): L (LPB + 1) complex, or reals when split
+    template <typename T>
+    static constexpr int exa() {
+        const int p = split_p<T>() ? (LP * (LPB_P + 1) + 1) / 2 : LP * (LPB_P + 1);
+        return p > LK * (LPB_K + 1) ? p : LK * (LPB_K + 1);
+    }
 };
 
-// LDS: exchange area (XReg::EXA complex) + tables (Poisson 2 * 2NX,
+// LDS: exchange area (XReg::exa complex units) + tables (Poisson 2 * 2NX,
 // kinetic 3 * NX + nz complex).
 // The kinetic tiles (twice the loads of a zero-padded Poisson tile) take the
 // first block indices so that they are dispatched first.
@@ -2129,11 +2181,11 @@ __global__ void __launch_bounds__(256) k_xpass_reg(LineArgs<T> a, int nblk_k) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
     using X = XReg<NX>;
-    cplx<T>* TB = E + X::EXA;
+    cplx<T>* TB = E + X::template exa<T>();
     if ((int)blockIdx.x < nblk_k)
         xline_reg<T, X::LK, X::LPB_K, false, X::AK>(a, blockIdx.x, E, TB);
     else
-        xline_reg<T, X::LP, X::LPB_P, true, X::AP>(a, blockIdx.x - nblk_k, E, TB);
+        xline_reg<T, X::LP, X::LPB_P, true, X::AP, X::template split_p<T>()>(a, blockIdx.x - nblk_k, E, TB);
 }
 
 // Register y passes of the pencil plane pass (compiled ny): the
@@ -2354,10 +2406,10 @@ __global__ void __launch_bounds__(256) k_xpass_reg_group(const LineArgs<T>* __re
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
     using X = XReg<NX>;
-    cplx<T>* TB = E + X::EXA;
+    cplx<T>* TB = E + X::template exa<T>();
     const int2 w = work[blockIdx.x];
     const LineArgs<T> a = args[w.x];
-    xline_reg<T, X::LP, X::LPB_P, true, X::AP>(a, w.y, E, TB);
+    xline_reg<T, X::LP, X::LPB_P, true, X::AP, X::template split_p<T>()>(a, w.y, E, TB);
 }
 
 // ---------------------------------------------------------------------------
@@ -3251,7 +3303,7 @@ static int launch_line_reg(gpe_handle* h, const LineArgs<T>& a, bool poisson, bo
     const int np = poisson ? (2 * h->p.ny * h->p.nz + X::LPB_P - 1) / X::LPB_P : 0;
     const int nk = kinetic ? (h->p.ny * h->p.nz + X::LPB_K - 1) / X::LPB_K : 0;
     if (np + nk == 0) return 0;
-    const size_t lds = (size_t)(X::EXA + 4 * NX) * sizeof(cplx<T>);
+    const size_t lds = (size_t)(X::template exa<T>() + 4 * NX) * sizeof(cplx<T>);
     hipEvent_t e0, e1;
     kernel_events(h, 1, &e0, &e1);
     hipExtLaunchKernelGGL((k_xpass_reg<T, NX>), dim3(np + nk), dim3(256), lds, h->stream, e0, e1, 0, a, nk);
@@ -3970,7 +4022,7 @@ static int batch_setup(gpe_batch* b) {
             int nt = la[r].ntile_p;
             if (reg) {
                 const int lpb = nx == 128 ? XReg<128>::LPB_P : nx == 64 ? XReg<64>::LPB_P : XReg<256>::LPB_P;
-                const int exa = nx == 128 ? XReg<128>::EXA : nx == 64 ? XReg<64>::EXA : XReg<256>::EXA;
+                const int exa = nx == 128 ? XReg<128>::exa<T>() : nx == 64 ? XReg<64>::exa<T>() : XReg<256>::exa<T>();
                 nt = (2 * hr->p.ny * hr->p.nz + lpb - 1) / lpb;
                 g.lds = (size_t)(exa + 4 * nx) * sizeof(cplx<T>);
             } else {
