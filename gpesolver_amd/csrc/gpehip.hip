@@ -123,6 +123,7 @@ struct PlaneArgs {
     cplx<T>* tko;       // written by M_KIN (== tk unless the buffers ping-pong)
     cplx<T>* s1o;       // written by M_RHO_FWD (== s1 unless the buffers ping-pong)
     int split;          // 1: two blocks per plane (grid 2 nx), see k_plane
+    int xcd;            // 1: contiguous plane ranges per XCD (IT, nx % 8 == 0), see k_plane
     int kz_skip;        // M_KIN leaves the z propagation to the x pass (LineArgs::kinz)
     const T* V;
     T* phi;
@@ -1954,6 +1955,16 @@ __global__ void __launch_bounds__(NT) k_plane(PlaneArgs<T> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double red[NT / 64 + 1];
     int i = blockIdx.x, mode = a.mode;
+    if (a.xcd) {
+        // Workgroups go to the eight XCDs in turn (block b to XCD b % 8), and
+        // each XCD has its own L2.  Block b = 8 s + x takes plane
+        // x (nx / 8) + s % (nx / 8), in role s / (nx / 8) of a split pass: XCD x
+        // holds planes [x nx / 8, (x + 1) nx / 8) in both roles, so the x
+        // neighbours an imaginary-time stencil reads, and the plane both roles
+        // read, are mostly in its L2.  Role A blocks still come first.
+        const int p8 = a.nx >> 3, x = blockIdx.x & 7, s = blockIdx.x >> 3;
+        i = (s / p8) * a.nx + x * p8 + s % p8;
+    }
     if (a.split) {
         if (i < a.nx) {
             mode &= ~M_RHO_FWD;
@@ -2899,6 +2910,7 @@ static double* energy_scratch(int dev) {
     return g_scratch[dev];
 }
 
+
 static int fail(gpe_handle* h, const std::string& msg) {
     if (h)
         h->err = msg;
@@ -3103,6 +3115,15 @@ static bool kz_in_x(const gpe_handle* h) {
     return false;
 }
 
+// XCD-aware plane order of k_plane (GPE_XCD_MAP=0 turns it off).
+static bool xcd_map_on() {
+    static const bool on = [] {
+        const char* e = getenv("GPE_XCD_MAP");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename T>
 static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     PlaneArgs<T> a;
@@ -3118,6 +3139,9 @@ static PlaneArgs<T> plane_args(gpe_handle* h, int mode) {
     a.tk = a.tko = (cplx<T>*)h->tk;
     a.s1 = a.s1o = (cplx<T>*)h->s1;
     a.split = 0;
+    // (imaginary time only: its stencil reads the neighbour planes; real time
+    // measured no gain)
+    a.xcd = (p.nx % 8 == 0 && (mode & M_IT) && xcd_map_on()) ? 1 : 0;
     a.kz_skip = kz_in_x(h) ? 1 : 0;
     a.V = (const T*)h->V;
     a.phi = (T*)h->phi;
@@ -4243,16 +4267,24 @@ static int batch_it(gpe_batch* b, long n) {
     return 0;
 }
 
-// Residual growth that sends a warm-started instance to its fallback state:
-// the residual ||H psi - mu psi|| exceeds kFallbackGrowth times the smallest
-// value it has had since the instance started.
-constexpr double kFallbackGrowth = 2.0;
+// Energy rise that sends a warm-started instance to its fallback state.  On
+// a stable explicit map imaginary time lowers E/N from one check to the next,
+// so E/N above the previous check's by more than the convergence tolerance
+// (kRiseTol relative when no tolerance is given) means the state is not
+// converging.  Nor is a state with dtau mu >= 1: the map's factor 1 - dtau mu
+// is not positive on it, so it is a mode an unstable map (dtau beyond the
+// stencil's limit) has grown, which E/N no longer changes on.  A growing
+// residual ||H psi - mu psi|| alone is not taken as a failure: it grows while
+// a state leaves a saddle point for a lower minimum, with E/N falling all the
+// way.
+constexpr double kRiseTol = 1e-9;
 
 template <typename T>
 static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double tol, long* steps_out,
                           double* e_out) {
     const int ni = (int)b->inst.size();
-    std::vector<double> last(ni, 0.0), rmin(ni, 0.0);
+    std::vector<double> last(ni, 0.0);
+    const double rise_tol = tol > 0 ? tol : kRiseTol;
     std::vector<char> have(ni, 0);
     b->fell_back.assign(ni, 0);
     b->active.clear();
@@ -4278,10 +4310,11 @@ static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double
             const double epa = e[5] / h->p.n_atoms, res = e[7];
             if (steps_out) steps_out[q] = steps;
             // fallback: non-finite state (e.g. a NaN or zero warm state that
-            // load_buffer could not normalise) or a growing residual
+            // load_buffer could not normalise) or a rising energy
             const bool bad = !std::isfinite(epa) || !std::isfinite(res);
-            const bool grew = have[q] && res > kFallbackGrowth * rmin[q];
-            if (!b->fallback[q].empty() && !b->fell_back[q] && (bad || grew)) {
+            const bool rose = have[q] && epa - last[q] > rise_tol * std::abs(epa);
+            const bool flip = e[6] * h->p.imag_time_dt >= 1.0;
+            if (!b->fallback[q].empty() && !b->fell_back[q] && (bad || rose || flip)) {
                 if (gpe_load_buffer(h, b->fallback[q].data())) return bfail(b, h->err);
                 b->fell_back[q] = 1;
                 have[q] = 0;
@@ -4291,7 +4324,6 @@ static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double
             if (e_out) e_out[q] = epa;
             const bool conv = tol > 0 && have[q] && std::abs(epa - last[q]) < tol * std::abs(epa);
             last[q] = epa;
-            rmin[q] = have[q] ? std::min(rmin[q], res) : res;
             have[q] = 1;
             if (!conv) keep.push_back(q);
         }

@@ -181,9 +181,12 @@ int gpe_batch_it_until(gpe_batch *b, long max_steps, long check_every, double to
  * psi (interleaved complex128, the StdMat3D layout; NULL clears it) is the
  * state gpe_batch_it_until restarts instance i from, once, when at a check
  * its energy or residual is not finite (a NaN or zero state given to
- * gpe_load_buffer) or its residual ||H psi - mu psi|| exceeds twice the
- * smallest value it had so far.  Its step count keeps running.
- * gpe_batch_fallback_taken tells whether the last gpe_batch_it_until took it. */
+ * gpe_load_buffer), its E/N has risen since the previous check by more
+ * than tol relative (1e-9 when tol is 0: imaginary time on a stable map only
+ * lowers it), or dtau mu >= 1 (the map's factor 1 - dtau mu is not positive
+ * on the state: a mode an unstable map has grown).  Its step count keeps
+ * running.  gpe_batch_fallback_taken tells whether the last
+ * gpe_batch_it_until took it. */
 int gpe_batch_set_fallback(gpe_batch *b, int i, const double *psi);
 int gpe_batch_fallback_taken(const gpe_batch *b, int i);
 

@@ -50,3 +50,32 @@ def test_fallback_on_nan_and_zero_warm_state():
     assert taken.tolist() == [True, True, False]
     np.testing.assert_array_equal(e_w, e_p)
     np.testing.assert_array_equal(s_w, s_p + np.array([100, 100, 0]))
+
+
+def test_fallback_on_energy_rise():
+    """An instance whose explicit imaginary-time map is unstable (dtau 6.3x
+    the reference value; the kinetic stencil alone has dtau lambda_max ~2.1
+    at 120 / 600 nm) grows a mode from round-off within the first 100 steps
+    and would then meet the |dE/N| test on it (E/N ~6e4 times the ground
+    state's); its dtau mu >= 1 sends it to its fallback.  The same instance at
+    the reference dtau lowers E/N at every check and keeps its warm state."""
+    import dataclasses
+
+    from gpesolver_amd import hip
+
+    rng = np.random.default_rng(11)
+    d = W.sample_instances(rng, 1)[0]
+    p, V, plain = W.build(d)
+    p_bad = dataclasses.replace(p, imag_time_dt=p.imag_time_dt * 6.3)
+    b = hip.Batch([p_bad.to_gpe(), p.to_gpe()], hip.GPE_FP32)
+    try:
+        for q, dev in enumerate(b.instances):
+            dev.init_potential(d["pot"], p)
+            dev.load_buffer(plain)
+            b.set_fallback(q, plain)
+        steps, e = b.it_until(1500, 100, 1e-6)
+        taken = b.fallbacks_taken()
+    finally:
+        b.close()
+    print("steps", steps, "E/N", e, "taken", taken, flush=True)
+    assert taken.tolist() == [True, False]
