@@ -58,18 +58,17 @@ __device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x
 __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 
 // Write-through store of one complex value, for the arrays one pass of a
-// step writes and the next pass reads (S1, Tk): global_store with the sc0 sc1
-// cache policy writes the line through the XCD's L2 instead of leaving it
-// dirty there.  The next pass reads it from blocks on other XCDs, whose L2s
-// hold no copy anyway, and a launch ends by writing back its XCD's dirty
-// lines; without them that write-back is short.  Measured at 128x128x32
-// FP32 real time: x pass 12.0 -> 9.4 us, plane pass 13.1 -> 12.8 us, against
-// nontemporal stores (plain stores: 14.5 us plane pass).  The FP32 store
-// moves 8 bytes (one dwordx2), the width up to which a vector store's data
-// registers may be rewritten right after it issues, which inline assembly
-// hides from the compiler's hazard checks.  An FP64 value would take two such
-// stores, and measured 10% slower than one nontemporal 16-byte store, which
-// FP64 keeps.
+// step writes and the next pass reads (S1, Tk, the IT state, the pencil
+// intermediates): global_store with the sc0 sc1 cache policy writes the line
+// through the XCD's L2 instead of leaving it dirty there.  The next pass reads
+// it from blocks on other XCDs, whose L2s hold no copy anyway, and a launch
+// ends by writing back its XCD's dirty lines; without them that write-back is
+// short.  Measured at 128x128x32 real time: FP32 x pass 12.0 -> 9.4 us, plane
+// pass 13.1 -> 12.8 us against nontemporal stores (plain stores: 14.5 us
+// plane pass); FP64 x pass 16.8 -> 15.2 us.  A vector store of more than 8
+// bytes needs one wait state before its data registers are rewritten, which
+// the compiler cannot see through inline assembly: the 16-byte FP64 store
+// carries its own s_nop.
 template <typename T>
 __device__ __forceinline__ void store_wt(cplx<T>* p, cplx<T> v) {
     typedef T v2 __attribute__((ext_vector_type(2)));
@@ -77,15 +76,7 @@ __device__ __forceinline__ void store_wt(cplx<T>* p, cplx<T> v) {
     if constexpr (sizeof(T) == 4)
         asm volatile("global_store_dwordx2 %0, %1, off sc0 sc1" ::"v"(p), "v"(t) : "memory");
     else
-        __builtin_nontemporal_store(t, reinterpret_cast<v2*>(p));
-}
-// The same for stores that FP64 keeps as plain stores.
-template <typename T>
-__device__ __forceinline__ void store_wt32(cplx<T>* p, cplx<T> v) {
-    if constexpr (sizeof(T) == 4)
-        store_wt(p, v);
-    else
-        *p = v;
+        asm volatile("global_store_dwordx4 %0, %1, off sc0 sc1\n\ts_nop 1" ::"v"(p), "v"(t) : "memory");
 }
 
 namespace {
@@ -1049,7 +1040,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 } else {
                     r = a.psii[plane_off + p];
                 }
-                if (!(mode & M_NO_PSI_OUT)) store_wt32(&a.out[plane_off + p], r);
+                if (!(mode & M_NO_PSI_OUT)) store_wt(&a.out[plane_off + p], r);
                 const T n2 = r.x * r.x + r.y * r.y;
                 rho_r[q] = n2;
                 nacc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
@@ -1748,7 +1739,7 @@ __global__ void __launch_bounds__(256) k_pencil_rz(PlaneArgs<T> a) {
             } else {
                 o = a.psii[e];
             }
-            store_wt32(&a.out[e], o);
+            store_wt(&a.out[e], o);
             c[q] = o;
             rho[q] = o.x * o.x + o.y * o.y;
             nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
@@ -1781,7 +1772,7 @@ __global__ void __launch_bounds__(256) k_pencil_rz(PlaneArgs<T> a) {
             c[q] = cc;
             rho[q] = nrm;  // the density before the phase, which leaves it unchanged
             nacc += (double)cc.x * (double)cc.x + (double)cc.y * (double)cc.y;
-            if (mode & M_OUT) store_wt32(&a.out[row_off + k], cc);
+            if (mode & M_OUT) store_wt(&a.out[row_off + k], cc);
         }
     }
     // ---- Poisson start: H = forward transform of the zero-padded density ---
@@ -1804,7 +1795,7 @@ __global__ void __launch_bounds__(256) k_pencil_rz(PlaneArgs<T> a) {
 #pragma unroll
             for (int k2 = 0; k2 < 5; ++k2) {
                 const int kz = k1 + 16 * k2;  // H holds kz = 1 .. nz
-                if (kz >= 1 && kz <= NZ) store_wt32(&hs[(size_t)j * nz + kz - 1], w[k2]);
+                if (kz >= 1 && kz <= NZ) store_wt(&hs[(size_t)j * nz + kz - 1], w[k2]);
             }
         }
         __syncthreads();
@@ -1838,7 +1829,7 @@ __global__ void __launch_bounds__(256) k_pencil_rz(PlaneArgs<T> a) {
         for (int k1 = 0; k1 < 8; ++k1) u[k1] = E[(k1 * B + b) * LS + r];
         gpe::ct::Dft<T, 8, +1, true>::run(u);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) store_wt32(&gw[(size_t)j * nz + 8 * q + b], u[q]);
+        for (int q = 0; q < 8; ++q) store_wt(&gw[(size_t)j * nz + 8 * q + b], u[q]);
     }
     if (mode & M_NORM_OUT) {
         const double sum = block_sum<256>(nacc, red);
@@ -2300,7 +2291,7 @@ __device__ __forceinline__ void yline_reg(const cplx<T>* in, size_t istr, int ni
 #pragma unroll
                 for (int r = 0; r < R; ++r)
 #pragma unroll
-                    for (int k2 = 0; k2 < B; ++k2) store_wt32(&out[(size_t)(b + B * r + A * k2) * ostr], w[r][k2]);
+                    for (int k2 = 0; k2 < B; ++k2) store_wt(&out[(size_t)(b + B * r + A * k2) * ostr], w[r][k2]);
             }
             return;
         }
@@ -2334,7 +2325,7 @@ __device__ __forceinline__ void yline_reg(const cplx<T>* in, size_t istr, int ni
     if (valid) {
 #pragma unroll
         for (int q = 0; q < A; ++q)
-            if (B * q + b < nout) store_wt32(&out[(size_t)(B * q + b) * ostr], v[q]);
+            if (B * q + b < nout) store_wt(&out[(size_t)(B * q + b) * ostr], v[q]);
     }
 }
 
