@@ -1118,7 +1118,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         }
         __syncthreads();
         if (mode & M_OUT) {
-            for (int p = threadIdx.x; p < npl; p += NT) a.out[plane_off + p] = Q[(p / nz) * qs + p % nz];
+            for (int p = threadIdx.x; p < npl; p += NT) store_wt(&a.out[plane_off + p], Q[(p / nz) * qs + p % nz]);
         }
         if ((mode & M_KIN) && !a.kz_skip) {
             if (mode & M_OUT) __syncthreads();  // the psi output above has read Q
@@ -1163,7 +1163,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
             Pol::template fft_y<T, NT, +1>(Q, nz, qs, 1, a, tw, twly);
             for (int p = threadIdx.x; p < npl; p += NT) {
                 const int j = p / nz, k = p - j * nz;
-                a.tko[((size_t)j * nx + i) * nz + k] = Q[j * qs + k];
+                store_wt(&a.tko[((size_t)j * nx + i) * nz + k], Q[j * qs + k]);
             }
         }
         __syncthreads();
@@ -1245,7 +1245,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         Pol::template fft_y2<T, NT, -1>(P + 1, nz, nzh, 1, a, tw);
         for (int e = threadIdx.x; e < ny2 * nz; e += NT) {
             const int ky = e / nz, c = e - ky * nz;
-            a.s1o[((size_t)ky * nx + i) * nz + c] = P[ky * nzh + c + 1];
+            store_wt(&a.s1o[((size_t)ky * nx + i) * nz + c], P[ky * nzh + c + 1]);
         }
     }
 }
@@ -1925,7 +1925,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
         Pol::template fft_x2<T, NT, +1>(X, C, CS, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            base[(size_t)ii * nzs + c] = X[ii * CS + c];
+            store_wt(&base[(size_t)ii * nzs + c], X[ii * CS + c]);
         }
     } else {
         b -= a.ntile_p;
@@ -1949,7 +1949,7 @@ __device__ __forceinline__ void line_body(const LineArgs<T>& a, int b, unsigned 
         Pol::template fft_x<T, NT, +1>(X, C, CS, 1, a);
         for (int e = threadIdx.x; e < nx * C; e += NT) {
             const int ii = e / C, c = e - ii * C;
-            base[(size_t)ii * nz + c] = X[ii * CS + c];
+            store_wt(&base[(size_t)ii * nz + c], X[ii * CS + c]);
         }
     }
 }
