@@ -279,6 +279,43 @@ def warm_vs_plain(n_train: int, n_test: int, epochs: int, seed: int = 20261203) 
     }
 
 
+def it_only(dev, args):
+    """IT leg alone on the default launch shapes: exactly --steps fused
+    imaginary-time steps, REPS times (median), then per-kernel event times.
+    Algorithmic bytes per IT launch (FP32): plane pass reads S1 + psi + V and
+    writes S1 + psi (52 N, as in real time with psi in place of Tk), x pass
+    reads and writes S1 (32 N)."""
+    nx, ny, nz = GRID
+    N = nx * ny * nz
+    dev.it_steps(args.warmup)
+    dev.synchronize()
+    reps = []
+    for _ in range(REPS):
+        dev.synchronize()
+        t0 = time.perf_counter()
+        dev.it_steps(args.steps)
+        dev.synchronize()
+        reps.append(time.perf_counter() - t0)
+    el = float(np.median(reps))
+    dev.set_timing(True)
+    dev.it_steps(min(args.steps, 1000))
+    tm = dev.timing()
+    dev.set_timing(False)
+    dev.close()
+    kb = {"plane": 52 * N, "line": 32 * N}
+    kern = {"plane": tm["plane_us"], "line": tm["line_us"]}
+    if args.calib_copy_mib > 0:
+        calib_copy(args.calib_copy_mib << 20)
+    H = 2 * nx * 2 * ny * (nz + 1)
+    print(json.dumps({"metric": "it_steps_per_s", "value": args.steps / el, "unit": "steps/s", "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "it_only": True,
+                      "avg_launch_us": kern, "algorithmic_bytes_per_launch": kb,
+                      "kernel_frac": {k: kb[k] / (kern[k] * 1e-6) / HBM_PEAK for k in kb},
+                      "step_model_bytes": 60 * N + 20 * H,
+                      "step_frac": (60 * N + 20 * H) * args.steps / el / HBM_PEAK,
+                      "calib_copy_mib": args.calib_copy_mib}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +333,10 @@ def main():
                     help="profiling runs: the RT leg only, on the default launch shapes (split plane pass, "
                          "register x pass: what the autotuner picks at this grid), no autotune and no IT legs, "
                          "so every steady-state launch in a trace or counter run is an RT one")
+    ap.add_argument("--it-only", action="store_true",
+                    help="profiling runs: the IT leg only (fused steps, default launch shapes), timed and "
+                         "per-kernel event times, so every steady-state launch in a trace or counter run is "
+                         "an IT one")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="end with one device-to-device copy of this many MiB (the known-size "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -317,6 +358,9 @@ def main():
 
     nx, ny, nz = GRID
     p, dev, V, psi = build_case(hip.GPE_FP32)
+    if args.it_only:
+        it_only(dev, args)
+        return
     # launch-shape variant for this grid (scratch instance)
     tuned = None if args.rt_only else dev.autotune(200)
     # warm-up
