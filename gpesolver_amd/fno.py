@@ -130,11 +130,9 @@ def features(p: P.PhysicalParameters, V: np.ndarray, pot: str) -> np.ndarray:
     n = V.size
     x = np.empty((n, CIN), dtype=np.float32)
     x[:, 0] = (V.reshape(-1) / V0).astype(np.float32)
-    x[:, 1] = (p.edd - 1.4) / 0.2
-    x[:, 2] = (math.log10(p.n_atoms) - 4.35) / 0.5
-    x[:, 3] = (P.freq_au_to_Hz(p.omega_z) - 120.0) / 20.0
-    x[:, 4] = 1.0 if pot.startswith("MEXICAN") else 0.0
-    x[:, 5] = P.len_au_to_nm(p.dd) / 1500.0
+    x[:, 1:] = np.array([(p.edd - 1.4) / 0.2, (math.log10(p.n_atoms) - 4.35) / 0.5,
+                         (P.freq_au_to_Hz(p.omega_z) - 120.0) / 20.0, 1.0 if pot.startswith("MEXICAN") else 0.0,
+                         P.len_au_to_nm(p.dd) / 1500.0], dtype=np.float32)
     return x
 
 

@@ -13,8 +13,8 @@ Pipeline:
      criterion.  Plain MULTIPLE_GAUSS imaginary time stays the fallback and
      the acceptance check (the warm run must reach the plain run's E/N): a
      non-finite or empty prediction is replaced before the run, and during the
-     run the batch restarts an instance from MULTIPLE_GAUSS when its state
-     turns non-finite or its residual ||H psi - mu psi|| grows
+     run the batch restarts an instance from MULTIPLE_GAUSS when its energy
+     turns non-finite or rises between checks, or dtau mu >= 1
      (gpe_batch_set_fallback).
 """
 from __future__ import annotations
@@ -46,11 +46,17 @@ def sample_instances(rng: np.random.Generator, n: int) -> list[dict]:
     return out
 
 
-def build(inst: dict, grid=GRID):
+def build_pote(inst: dict, grid=GRID):
+    """The instance's parameters and potential (what the warm start's
+    features need)."""
     nx, ny, nz = grid
     p = P.PhysicalParameters.from_physical(nx, ny, nz, *SPACING_NM, n_atoms=inst["n_atoms"], edd=inst["edd"],
                                            omega_hz=(60.0, 60.0, inst["omega_z"]), dd_nm=inst["dd_nm"])
-    V = P.potential(inst["pot"], p)
+    return p, P.potential(inst["pot"], p)
+
+
+def build(inst: dict, grid=GRID):
+    p, V = build_pote(inst, grid)
     # MULTIPLE_GAUSS(2) placement with a 4-cell width: on 64x64x16 the
     # reference width L/20 is 0.8 cells along z, and the explicit imaginary-time
     # map then collapses the state onto one grid point within ~100 steps.
@@ -129,7 +135,7 @@ def predict_states(model: F.Fno, insts: list[dict], grid=GRID, batch=8):
     prediction.  Returns (states, fallback flags)."""
     feats = []
     for d in insts:
-        p, V, _ = build(d, grid)
+        p, V = build_pote(d, grid)
         feats.append(F.features(p, V, d["pot"]))
     feats = np.stack(feats)
     states, fb = [], []
