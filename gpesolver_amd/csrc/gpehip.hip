@@ -2437,6 +2437,67 @@ __global__ void __launch_bounds__(256) k_xpass_reg_group(const LineArgs<T>* __re
     xline_reg<T, X::LP, X::LPB_P, true, X::AP, X::template split_p<T>()>(a, w.y, E, TB);
 }
 
+// One launch per pass over the instances of several grid shapes (a batch):
+// work item (instance, index | slot << kSlotShift), where slot names the
+// compiled kernel body that item runs.  Every compiled FP32 plane shape runs
+// blocks of the same size within 128 VGPRs, so one launch carries them all
+// and their blocks interleave on the GPU; the x pass likewise over the
+// register Poisson tiles of nx = 128 and 64 and the LDS line shapes below
+// 256 (nx = 256 needs more VGPRs, and a kernel's register count is that of
+// its largest body).  FP32 only: in FP64 the joint plane body needs more
+// registers than its 512-thread blocks have and spills.
+constexpr int kSlotShift = 20;
+constexpr int kSlotMask = (1 << kSlotShift) - 1;
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_plane_multi(const PlaneArgs<T>* __restrict__ args,
+                                                    const int2* __restrict__ work) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[NT / 64 + 1];
+    const int2 w = work[blockIdx.x];
+    const PlaneArgs<T> a = args[w.x];
+    const int i = w.y & kSlotMask, slot = w.y >> kSlotShift;
+    int s = 0;
+#define GPE_PM(NY, NZ)                                                        \
+    if (slot == s++) {                                                      \
+        plane_body<T, NT, PlanePolCT<NY, NZ>>(a, i, a.mode, lds_raw, red);  \
+        return;                                                             \
+    }
+    GPE_PLANE_SHAPES(GPE_PM)
+#undef GPE_PM
+}
+
+// x-pass slots: 0 and 1 the register Poisson tiles of nx = 128 and 64, then
+// the LDS pass of each x length of kMultiLineShapes.
+#define GPE_MULTI_LINE_SHAPES(X) X(128) X(64) X(96) X(16) X(24) X(32)
+template <typename T>
+__global__ void __launch_bounds__(kLineThreads) k_line_multi(const LineArgs<T>* __restrict__ args,
+                                                            const int2* __restrict__ work) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int2 w = work[blockIdx.x];
+    const LineArgs<T> a = args[w.x];
+    const int t = w.y & kSlotMask, slot = w.y >> kSlotShift;
+    if (slot < 2) {
+        cplx<T>* E = reinterpret_cast<cplx<T>*>(lds_raw);
+        if (slot == 0) {
+            using X = XReg<128>;
+            xline_reg<T, X::LP, X::LPB_P, true, X::AP, X::template split_p<T>()>(a, t, E, E + X::template exa<T>());
+        } else {
+            using X = XReg<64>;
+            xline_reg<T, X::LP, X::LPB_P, true, X::AP, X::template split_p<T>()>(a, t, E, E + X::template exa<T>());
+        }
+        return;
+    }
+    int s = 2;
+#define GPE_LM(NX)                                                    \
+    if (slot == s++) {                                              \
+        line_body<T, kLineThreads, LinePolCT<NX>>(a, t, lds_raw);   \
+        return;                                                     \
+    }
+    GPE_MULTI_LINE_SHAPES(GPE_LM)
+#undef GPE_LM
+}
+
 // ---------------------------------------------------------------------------
 // Elementwise / reduction kernels.
 
@@ -3956,6 +4017,14 @@ struct gpe_batch {
         size_t lds;
     };
     std::vector<Group> pgroups, lgroups;
+    // one launch per pass over every shape of the active set (k_plane_multi,
+    // k_line_multi) when it has more than one and all have those bodies:
+    // work items carry the shape slot
+    bool multi_p = false, multi_l = false;
+    void* d_pmwork = nullptr;
+    void* d_lmwork = nullptr;
+    int npmwork = 0, nlmwork = 0;
+    size_t cap_pmwork = 0, cap_lmwork = 0, lds_pm = 0, lds_lm = 0;
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     bool use_graphs = true;
     // Device buffers sized at the first setup for every instance and kept
@@ -3969,7 +4038,7 @@ struct gpe_batch {
     double* d_esum = nullptr;  // per instance: sums[8], out[8]
     double* h_eout = nullptr;  // pinned: out[8] per active instance
     // Fallback states (gpe_batch_set_fallback): the state an instance
-    // restarts from when its own is non-finite or its residual grows.
+    // restarts from when its energy is non-finite or rises, or dtau mu >= 1.
     std::vector<std::vector<double>> fallback;
     std::vector<char> fell_back;
 };
@@ -3997,6 +4066,36 @@ static void batch_drop_graphs(gpe_batch* b) {
 // Argument blocks and work lists for the current active set.  Slots:
 // [0..3] prologue, [4..7] steady step, [8..11] last step, indexed by parity
 // (cur*2 + pcur).
+// Slot of a plane shape in k_plane_multi (-1: no compiled body).
+static int plane_slot(int ny, int nz) {
+    int s = 0, r = -1;
+#define GPE_PS(NY, NZ)                     \
+    if (r < 0 && ny == NY && nz == NZ) r = s; \
+    ++s;
+    GPE_PLANE_SHAPES(GPE_PS)
+#undef GPE_PS
+    return r;
+}
+// Slot of an x-pass group in k_line_multi (-1: none).
+static int line_slot(int kind, int nx) {
+    if (kind == 1) return nx == 128 ? 0 : nx == 64 ? 1 : -1;
+    int s = 2, r = -1;
+#define GPE_LS(NX)                 \
+    if (r < 0 && nx == NX) r = s; \
+    ++s;
+    GPE_MULTI_LINE_SHAPES(GPE_LS)
+#undef GPE_LS
+    return r;
+}
+// GPE_BATCH_MULTI=0: one launch per grid shape and pass.
+static bool batch_multi_on() {
+    static const bool on = [] {
+        const char* e = getenv("GPE_BATCH_MULTI");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename T>
 static int batch_setup(gpe_batch* b) {
     batch_drop_graphs(b);
@@ -4071,6 +4170,45 @@ static int batch_setup(gpe_batch* b) {
         g.cnt = (int)lw.size() - g.off;
         b->lgroups.push_back(g);
     }
+    // the single-launch work lists, when the active set spans several shapes
+    std::vector<int2> pm, lm;
+    b->multi_p = b->multi_l = false;
+    if (sizeof(T) == 4 && !gen && batch_multi_on()) {  // FP64: the joint bodies spill
+        bool ok = b->pgroups.size() > 1;
+        size_t lds = 0;
+        for (const auto& g : b->pgroups) {
+            const int slot = plane_slot(g.a, g.b);
+            if (slot < 0) ok = false;
+            for (int e = 0; ok && e < g.cnt; ++e) {
+                const int2 v = pw[(size_t)g.off + e];
+                pm.push_back(make_int2(v.x, v.y | (slot << kSlotShift)));
+            }
+            lds = std::max(lds, g.lds);
+        }
+        if (ok) {
+            b->multi_p = true;
+            b->lds_pm = lds;
+        } else {
+            pm.clear();
+        }
+        ok = b->lgroups.size() > 1;
+        lds = 0;
+        for (const auto& g : b->lgroups) {
+            const int slot = line_slot(g.kind, g.a);
+            if (slot < 0) ok = false;
+            for (int e = 0; ok && e < g.cnt; ++e) {
+                const int2 v = lw[(size_t)g.off + e];
+                lm.push_back(make_int2(v.x, v.y | (slot << kSlotShift)));
+            }
+            lds = std::max(lds, g.lds);
+        }
+        if (ok) {
+            b->multi_l = true;
+            b->lds_lm = lds;
+        } else {
+            lm.clear();
+        }
+    }
     // energy check: per-instance arguments, (instance, chunk) and (instance,
     // plane) work lists
     std::vector<EArgs<T>> ea(na);
@@ -4122,6 +4260,8 @@ static int batch_setup(gpe_batch* b) {
     };
     b->npwork = (int)pw.size();
     b->nlwork = (int)lw.size();
+    b->npmwork = (int)pm.size();
+    b->nlmwork = (int)lm.size();
     b->nswork = (int)sw.size();
     b->nework = (int)ew.size();
     if (na == 0) return 0;
@@ -4131,7 +4271,9 @@ static int batch_setup(gpe_batch* b) {
         ensure(&b->d_lwork, b->cap_lwork, lw.size() * sizeof(int2)) ||
         ensure(&b->d_eargs, b->cap_eargs, ea.size() * sizeof(EArgs<T>)) ||
         ensure(&b->d_swork, b->cap_swork, sw.size() * sizeof(int2)) ||
-        ensure(&b->d_ework, b->cap_ework, ew.size() * sizeof(int2)))
+        ensure(&b->d_ework, b->cap_ework, ew.size() * sizeof(int2)) ||
+        ensure(&b->d_pmwork, b->cap_pmwork, pm.size() * sizeof(int2)) ||
+        ensure(&b->d_lmwork, b->cap_lmwork, lm.size() * sizeof(int2)))
         return 1;
     BCHECK(hipMemcpy(b->d_pargs, pa.data(), pa.size() * sizeof(PlaneArgs<T>), hipMemcpyHostToDevice));
     BCHECK(hipMemcpy(b->d_largs, la.data(), la.size() * sizeof(LineArgs<T>), hipMemcpyHostToDevice));
@@ -4140,6 +4282,8 @@ static int batch_setup(gpe_batch* b) {
     BCHECK(hipMemcpy(b->d_eargs, ea.data(), ea.size() * sizeof(EArgs<T>), hipMemcpyHostToDevice));
     BCHECK(hipMemcpy(b->d_swork, sw.data(), sw.size() * sizeof(int2), hipMemcpyHostToDevice));
     BCHECK(hipMemcpy(b->d_ework, ew.data(), ew.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!pm.empty()) BCHECK(hipMemcpy(b->d_pmwork, pm.data(), pm.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!lm.empty()) BCHECK(hipMemcpy(b->d_lmwork, lm.data(), lm.size() * sizeof(int2), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -4173,6 +4317,15 @@ static int batch_check(gpe_batch* b, double* out_host) {
 template <typename T>
 static int batch_plane(gpe_batch* b, int slot) {
     const PlaneArgs<T>* args = (const PlaneArgs<T>*)b->d_pargs + (size_t)slot * b->active.size();
+    if constexpr (sizeof(T) == 4) {
+        if (b->multi_p) {
+            constexpr int NT = PlaneThreads<T, true>::value;
+            hipLaunchKernelGGL((k_plane_multi<T, NT>), dim3(b->npmwork), dim3(NT), b->lds_pm, b->stream, args,
+                               (const int2*)b->d_pmwork);
+            BCHECK(hipGetLastError());
+            return 0;
+        }
+    }
     const bool gen = b->inst[0]->force_generic;
     for (const auto& g : b->pgroups) {
         const int2* w = (const int2*)b->d_pwork + g.off;
@@ -4201,6 +4354,14 @@ static int batch_plane(gpe_batch* b, int slot) {
 template <typename T>
 static int batch_line(gpe_batch* b) {
     const LineArgs<T>* args = (const LineArgs<T>*)b->d_largs;
+    if constexpr (sizeof(T) == 4) {
+        if (b->multi_l) {
+            hipLaunchKernelGGL((k_line_multi<T>), dim3(b->nlmwork), dim3(kLineThreads), b->lds_lm, b->stream, args,
+                               (const int2*)b->d_lmwork);
+            BCHECK(hipGetLastError());
+            return 0;
+        }
+    }
     const bool gen = b->inst[0]->force_generic;
     for (const auto& g : b->lgroups) {
         const int2* w = (const int2*)b->d_lwork + g.off;
@@ -4756,7 +4917,8 @@ void gpe_batch_destroy(gpe_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graphs(b);
     for (gpe_handle* h : b->inst) gpe_destroy(h);
-    void* bufs[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork, b->d_eargs, b->d_swork, b->d_ework, b->d_esum};
+    void* bufs[] = {b->d_pargs, b->d_largs, b->d_pwork, b->d_lwork,  b->d_eargs,
+                    b->d_swork, b->d_ework, b->d_esum,  b->d_pmwork, b->d_lmwork};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (b->h_eout) (void)hipHostFree(b->h_eout);
