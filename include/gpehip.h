@@ -160,10 +160,11 @@ int gpe_imag_time_until(gpe_handle *h, long max_steps, long check_every, double 
 /* Batch (no reference counterpart: the reference is single-instance with
  * singleton parameters, src/parameters/parameters.hpp:14-20).  n instances,
  * each with its own constants and grid (grids may differ), share one stream;
- * every pass of a step is one launch per grid shape over all active
- * instances of that shape (the single-instance kernels, so results equal a
- * single handle's bit for bit), and the steady-state steps are replayed as
- * hipGraphs.  gpe_batch_instance gives the per-instance handle for
+ * in FP32 every pass of a step is one launch over all active instances, each
+ * block running the single-instance kernel body of its instance's grid (so
+ * results equal a single handle's bit for bit; one launch per grid shape in
+ * FP64 and for run-time-size grids), and the steady-state steps are replayed
+ * as hipGraphs.  gpe_batch_instance gives the per-instance handle for
  * load/export/energy calls. */
 typedef struct gpe_batch gpe_batch;
 int gpe_batch_create(const gpe_params *params, int n, int precision, gpe_batch **out);
