@@ -201,6 +201,19 @@ Args mkargs(int M, int N, int K, int batch) {
 // is handled by callers): C[M x N] = A[M x K] . B[K x N] (+bias) (+beta C)
 int gemm_rm(gpe_fno* f, const float* A, const float* B, float* C, long M, int N, int K, bool transA, bool transB,
             const float* bias, float beta) {
+    if (!transA && N <= 64 && K <= 64 && ((uintptr_t)A % 16) == 0) {
+        // many rows, small N and K: the row GEMM (same results as k_gemm)
+        const long waves = ((M + 15) / 16 + gpe::gemm::kRowGroups - 1) / gpe::gemm::kRowGroups;
+        const dim3 grid((unsigned)((waves + 3) / 4));
+        if (transB)
+            hipLaunchKernelGGL(gpe::gemm::k_gemm_rows<true>, grid, dim3(256), 0, f->stream, A, B, C, M, N, K, bias,
+                               1.f, beta);
+        else
+            hipLaunchKernelGGL(gpe::gemm::k_gemm_rows<false>, grid, dim3(256), 0, f->stream, A, B, C, M, N, K, bias,
+                               1.f, beta);
+        FCHECK(hipGetLastError());
+        return 0;
+    }
     // one launch per 2^21 rows (grid.y = 32768 tiles of 64 rows at most)
     const long chunk = 1L << 21;
     for (long m0 = 0; m0 < M; m0 += chunk * 1) {
@@ -857,6 +870,56 @@ int gpe_fno_adam_step(gpe_fno* f, float lr) {
     FCHECK(hipGetLastError());
     FCHECK(hipStreamSynchronize(f->stream));
     return 0;
+}
+
+// Row GEMM entry used by the tests: C = A op(B) (+ bias) + beta C on the row
+// kernel (rows = 1) or the tile kernel (rows = 0), same operands; host
+// row-major matrices, B [K][N] or (transB) [N][K], C in / out.
+int gpe_gemm_rows_test(const float* A, const float* B, const float* bias, float* C, long M, int N, int K, int transB,
+                       float beta, int rows) {
+    if (M <= 0 || N <= 0 || N > 64 || K <= 0 || K > 64) return 1;
+    float *da = nullptr, *db = nullptr, *dc = nullptr, *dbias = nullptr;
+    int rc = 0;
+    rc |= hipMalloc(&da, (size_t)M * K * 4) != hipSuccess;
+    rc |= hipMalloc(&db, (size_t)K * N * 4) != hipSuccess;
+    rc |= hipMalloc(&dc, (size_t)M * N * 4) != hipSuccess;
+    if (bias) rc |= hipMalloc(&dbias, (size_t)N * 4) != hipSuccess;
+    if (!rc) {
+        (void)hipMemcpy(da, A, (size_t)M * K * 4, hipMemcpyHostToDevice);
+        (void)hipMemcpy(db, B, (size_t)K * N * 4, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dc, C, (size_t)M * N * 4, hipMemcpyHostToDevice);
+        if (bias) (void)hipMemcpy(dbias, bias, (size_t)N * 4, hipMemcpyHostToDevice);
+        if (rows) {
+            const long waves = ((M + 15) / 16 + gpe::gemm::kRowGroups - 1) / gpe::gemm::kRowGroups;
+            const dim3 grid((unsigned)((waves + 3) / 4));
+            if (transB)
+                hipLaunchKernelGGL(gpe::gemm::k_gemm_rows<true>, grid, dim3(256), 0, 0, da, db, dc, M, N, K, dbias,
+                                   1.f, beta);
+            else
+                hipLaunchKernelGGL(gpe::gemm::k_gemm_rows<false>, grid, dim3(256), 0, 0, da, db, dc, M, N, K, dbias,
+                                   1.f, beta);
+        } else {
+            Args a = mkargs((int)M, N, K, 1);
+            a.Ar = da;
+            a.sam = K;
+            a.sak = 1;
+            a.Br = db;
+            a.sbk = transB ? 1 : N;
+            a.sbn = transB ? K : 1;
+            a.Cr = dc;
+            a.scm = N;
+            a.scn = 1;
+            a.bias = dbias;
+            a.beta = beta;
+            dim3 grid((N + 63) / 64, (unsigned)((M + 63) / 64), 1);
+            hipLaunchKernelGGL(gpe::gemm::k_gemm, grid, dim3(256), 0, 0, a);
+        }
+        rc = hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess;
+        if (!rc) (void)hipMemcpy(C, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost);
+    }
+    for (float* p : {da, db, dc, dbias})
+        if (p) (void)hipFree(p);
+    return rc;
 }
 
 // Dense GEMM entry used by the tests (C = A B, all row-major, real or complex).

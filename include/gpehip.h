@@ -220,6 +220,12 @@ int gpe_fno_adam_step(gpe_fno *f, float lr);
  * may be NULL; Ci NULL keeps only the real part).  Test entry point. */
 int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float *Bi, float *Cr, float *Ci, int M,
                   int N, int K, int conjA, int conjB);
+/* C = A op(B) (+ bias) + beta C for many rows and N, K <= 64 (the network's
+ * pointwise layers) on the row kernel (rows = 1) or the tile kernel
+ * (rows = 0); host row-major matrices, B [K][N] or (transB) [N][K], C in /
+ * out.  Test entry point: the two kernels agree bit for bit. */
+int gpe_gemm_rows_test(const float *A, const float *B, const float *bias, float *C, long M, int N, int K, int transB,
+                       float beta, int rows);
 
 /* Autotuner (no reference counterpart).  Times the launch-shape variants of
  * the fused loops (one or two blocks per plane in the steady-state plane pass;

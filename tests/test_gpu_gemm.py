@@ -76,3 +76,36 @@ def test_mixed_real_complex_and_real_part_only():
     ref = ((bf16(A.real) + 1j * bf16(A.imag)).astype(np.complex128)
            @ (bf16(Bc.real) + 1j * bf16(Bc.imag)).astype(np.complex128)).real
     assert np.abs(Cr - ref).max() < 1e-4 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("transB", [0, 1])
+@pytest.mark.parametrize("M,N,K,beta,with_bias", [(1000, 32, 32, 1.0, True), (777, 64, 32, 0.0, True),
+                                                  (1024, 1, 64, 0.0, True), (513, 64, 1, 0.0, False),
+                                                  (300, 32, 6, 0.0, True), (4096, 32, 64, 0.5, False)])
+def test_row_gemm_equals_tile_gemm(M, N, K, beta, with_bias, transB):
+    """The row GEMM of the network's pointwise layers (many rows, N, K <= 64)
+    gives the tile kernel's results bit for bit: same BF16 operand rounding,
+    k order and epilogue; and it is the BF16 product to FP32 accumulation."""
+    L = hip.load()
+    fp = C.POINTER(C.c_float)
+    L.gpe_gemm_rows_test.argtypes = [fp, fp, fp, fp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int]
+    L.gpe_gemm_rows_test.restype = C.c_int
+    rng = np.random.default_rng(M + 3 * N + 7 * K + transB)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if transB else (K, N)).astype(np.float32)
+    bias = rng.standard_normal(N).astype(np.float32) if with_bias else None
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    out = []
+    for rows in (0, 1):
+        Cm = C0.copy()
+        rc = L.gpe_gemm_rows_test(A.ctypes.data_as(fp), B.ctypes.data_as(fp),
+                                  bias.ctypes.data_as(fp) if bias is not None else None, Cm.ctypes.data_as(fp), M, N,
+                                  K, transB, beta, rows)
+        assert rc == 0
+        out.append(Cm)
+    np.testing.assert_array_equal(out[1], out[0])
+    Bk = B.T if transB else B
+    ref = bf16(A).astype(np.float64) @ bf16(Bk).astype(np.float64) + beta * C0.astype(np.float64)
+    if bias is not None:
+        ref += bias.astype(np.float64)
+    assert np.abs(out[1] - ref).max() < 1e-4 * np.abs(ref).max() + 1e-5
