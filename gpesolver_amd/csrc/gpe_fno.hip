@@ -153,6 +153,11 @@ struct gpe_fno {
     float *dh, *dz, *dq, *dy, *dtmp;       // backward scratch
     float* kpart = nullptr;                // split-K partials
     double* lpart = nullptr;
+    // training set resident on the device (gpe_fno_set_data): inputs
+    // [n][P][cin], targets [n][P]
+    float* data_x = nullptr;
+    float* data_t = nullptr;
+    long data_n = 0;
     hipStream_t stream = nullptr;
     std::string err;
 };
@@ -811,7 +816,8 @@ void gpe_fno_destroy(gpe_fno* f) {
     if (!f) return;
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     std::vector<void*> bufs = {f->w,  f->g,     f->am,    f->av, f->xin, f->c1r, f->c1i, f->c2r, f->c2i, f->q_pre,
-                               f->q,  f->y,     f->t,     f->dh, f->dz,  f->dq,  f->dy,  f->dtmp, f->kpart, f->lpart};
+                               f->q,  f->y,     f->t,     f->dh, f->dz,  f->dq,  f->dy,  f->dtmp, f->kpart, f->lpart,
+                               f->data_x, f->data_t};
     for (auto* p : f->h) bufs.push_back(p);
     for (auto* p : f->zz) bufs.push_back(p);
     for (auto* p : f->s3r) bufs.push_back(p);
@@ -827,16 +833,23 @@ void gpe_fno_destroy(gpe_fno* f) {
 const char* gpe_fno_last_error(const gpe_fno* f) { return f ? f->err.c_str() : ""; }
 long gpe_fno_num_params(const gpe_fno* f) { return f ? (long)f->nparam : 0; }
 
+// Host copies of the parameters and gradients go through the network's own
+// stream: it is non-blocking, so a copy on the null stream would not wait for
+// the launches still running on it (a backward pass enqueued by
+// gpe_fno_loss_grad writes its last gradients after that call returns).
 int gpe_fno_get_params(gpe_fno* f, float* host) {
-    FCHECK(hipMemcpy(host, f->w, f->nparam * 4, hipMemcpyDeviceToHost));
+    FCHECK(hipMemcpyAsync(host, f->w, f->nparam * 4, hipMemcpyDeviceToHost, f->stream));
+    FCHECK(hipStreamSynchronize(f->stream));
     return 0;
 }
 int gpe_fno_set_params(gpe_fno* f, const float* host) {
-    FCHECK(hipMemcpy(f->w, host, f->nparam * 4, hipMemcpyHostToDevice));
+    FCHECK(hipMemcpyAsync(f->w, host, f->nparam * 4, hipMemcpyHostToDevice, f->stream));
+    FCHECK(hipStreamSynchronize(f->stream));
     return 0;
 }
 int gpe_fno_get_grads(gpe_fno* f, float* host) {
-    FCHECK(hipMemcpy(host, f->g, f->nparam * 4, hipMemcpyDeviceToHost));
+    FCHECK(hipMemcpyAsync(host, f->g, f->nparam * 4, hipMemcpyDeviceToHost, f->stream));
+    FCHECK(hipStreamSynchronize(f->stream));
     return 0;
 }
 
@@ -856,6 +869,39 @@ int gpe_fno_loss_grad(gpe_fno* f, const float* x, const float* target, int B, do
     if (((long)B * f->P) % 8192) return ffail(f, "batch*points must be a multiple of 8192");
     FCHECK(hipMemcpyAsync(f->xin, x, (size_t)B * f->P * f->cin * 4, hipMemcpyHostToDevice, f->stream));
     FCHECK(hipMemcpyAsync(f->t, target, (size_t)B * f->P * 4, hipMemcpyHostToDevice, f->stream));
+    if (fno_forward(f, B)) return 1;
+    if (fno_loss(f, B, true, loss)) return 1;
+    return fno_backward(f, B);
+}
+
+int gpe_fno_set_data(gpe_fno* f, const float* x, const float* target, long n) {
+    if (!f || n <= 0 || !x || !target) return ffail(f, "bad training set");
+    FCHECK(hipStreamSynchronize(f->stream));  // no launch may still read the old set
+    if (f->data_x) FCHECK(hipFree(f->data_x));
+    if (f->data_t) FCHECK(hipFree(f->data_t));
+    f->data_x = f->data_t = nullptr;
+    f->data_n = 0;
+    const size_t nx = (size_t)n * f->P * f->cin, nt = (size_t)n * f->P;
+    FCHECK(hipMalloc(&f->data_x, nx * 4));
+    FCHECK(hipMalloc(&f->data_t, nt * 4));
+    FCHECK(hipMemcpyAsync(f->data_x, x, nx * 4, hipMemcpyHostToDevice, f->stream));
+    FCHECK(hipMemcpyAsync(f->data_t, target, nt * 4, hipMemcpyHostToDevice, f->stream));
+    FCHECK(hipStreamSynchronize(f->stream));
+    f->data_n = n;
+    return 0;
+}
+
+int gpe_fno_loss_grad_idx(gpe_fno* f, const int* idx, int B, double* loss) {
+    if (!f || B <= 0 || B > f->maxb || !idx) return ffail(f, "bad batch size");
+    if (((long)B * f->P) % 8192) return ffail(f, "batch*points must be a multiple of 8192");
+    const size_t sx = (size_t)f->P * f->cin * 4, st = (size_t)f->P * 4;
+    for (int q = 0; q < B; ++q) {
+        if (idx[q] < 0 || idx[q] >= f->data_n) return ffail(f, "sample index outside the training set");
+        FCHECK(hipMemcpyAsync((char*)f->xin + q * sx, (const char*)f->data_x + idx[q] * sx, sx,
+                              hipMemcpyDeviceToDevice, f->stream));
+        FCHECK(hipMemcpyAsync((char*)f->t + q * st, (const char*)f->data_t + idx[q] * st, st,
+                              hipMemcpyDeviceToDevice, f->stream));
+    }
     if (fno_forward(f, B)) return 1;
     if (fno_loss(f, B, true, loss)) return 1;
     return fno_backward(f, B);

@@ -51,6 +51,10 @@ def _bind():
     L.gpe_fno_loss_grad.restype = ip
     L.gpe_fno_adam_step.argtypes = [vp, C.c_float]
     L.gpe_fno_adam_step.restype = ip
+    L.gpe_fno_set_data.argtypes = [vp, fp, fp, C.c_long]
+    L.gpe_fno_set_data.restype = ip
+    L.gpe_fno_loss_grad_idx.argtypes = [vp, C.POINTER(C.c_int), ip, dp]
+    L.gpe_fno_loss_grad_idx.restype = ip
     return L
 
 
@@ -119,6 +123,21 @@ class Fno:
         t = np.ascontiguousarray(t, dtype=np.float32)
         loss = C.c_double()
         self._check(self.L.gpe_fno_loss_grad(self.f, _fp(x), _fp(t), x.shape[0], C.byref(loss)))
+        return loss.value
+
+    def set_data(self, x: np.ndarray, t: np.ndarray):
+        """Keep the training set [n, P, cin] / [n, P] on the device."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        assert x.shape[0] == t.shape[0]
+        self._check(self.L.gpe_fno_set_data(self.f, _fp(x), _fp(t), x.shape[0]))
+
+    def loss_grad_idx(self, idx) -> float:
+        """loss_grad on samples idx of the device-resident training set."""
+        ii = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_double()
+        self._check(self.L.gpe_fno_loss_grad_idx(self.f, ii.ctypes.data_as(C.POINTER(C.c_int)), ii.size,
+                                                 C.byref(loss)))
         return loss.value
 
     def adam_step(self, lr: float):

@@ -112,8 +112,11 @@ def generate(insts: list[dict], tol=1e-6, max_steps=5000, chunk=64, log=print, g
 
 
 def train(model: F.Fno, X: np.ndarray, T: np.ndarray, epochs: int, lr: float, batch: int, log=print, seed=0):
+    """Adam with a cosine learning rate over `epochs` passes of shuffled
+    batches; the training set stays on the device for the whole run."""
     rng = np.random.default_rng(seed)
     n = X.shape[0]
+    model.set_data(X, T)
     hist = []
     for ep in range(epochs):
         perm = rng.permutation(n)
@@ -122,7 +125,7 @@ def train(model: F.Fno, X: np.ndarray, T: np.ndarray, epochs: int, lr: float, ba
         lr_ep = lr * 0.5 * (1 + math.cos(math.pi * ep / max(1, epochs)))
         for b0 in range(0, n - batch + 1, batch):
             idx = perm[b0:b0 + batch]
-            tot += model.loss_grad(X[idx], T[idx])
+            tot += model.loss_grad_idx(idx)
             model.adam_step(lr_ep)
             nb += 1
         hist.append(tot / max(1, nb))

@@ -216,6 +216,12 @@ int gpe_fno_get_grads(gpe_fno *f, float *host);
 int gpe_fno_forward(gpe_fno *f, const float *x, int batch, float *y_out);
 int gpe_fno_loss_grad(gpe_fno *f, const float *x, const float *target, int batch, double *loss);
 int gpe_fno_adam_step(gpe_fno *f, float lr);
+/* Training set resident on the device: n samples, inputs [n][P][cin] and
+ * targets [n][P] (host, FP32), copied once; gpe_fno_loss_grad_idx then takes
+ * the batch idx[0..batch) from it with device copies, as gpe_fno_loss_grad
+ * does from host arrays. */
+int gpe_fno_set_data(gpe_fno *f, const float *x, const float *target, long n);
+int gpe_fno_loss_grad_idx(gpe_fno *f, const int *idx, int batch, double *loss);
 /* C = op(A) op(B) on the MFMA GEMM (row-major host matrices; imaginary parts
  * may be NULL; Ci NULL keeps only the real part).  Test entry point. */
 int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float *Bi, float *Cr, float *Ci, int M,

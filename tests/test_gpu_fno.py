@@ -101,3 +101,20 @@ def test_config4_size_forward_loss_and_gradients():
     assert cos > 0.99
     assert np.median(rel) < 0.05 and rel.max() < 0.3
     m.close()
+
+
+def test_device_training_set_matches_host_batches():
+    """loss_grad_idx on the device-resident training set equals loss_grad on
+    the same samples passed from the host, loss and gradients bit for bit
+    (the batch is the same inputs, copied on the device)."""
+    m, ref, x, t = make(9)
+    t = t * np.linspace(0.5, 1.5, 4, dtype=np.float32)[:, None]
+    m.set_data(x, t)
+    idx = np.array([3, 1, 2, 0], dtype=np.int32)  # 4 x 2048 points: the 8192-point granule of the loss
+    l_dev = m.loss_grad_idx(idx)
+    g_dev = m.get_grads()
+    l_host = m.loss_grad(x[idx], t[idx])
+    g_host = m.get_grads()
+    assert l_dev == l_host
+    np.testing.assert_array_equal(g_dev, g_host)
+    m.close()
