@@ -253,7 +253,25 @@ __global__ void __launch_bounds__(256) k_gemm_small(Args a) {
 // the group's 16 x N outputs take one MFMA 16x16x32 per 16 columns and 32 k.
 // Operand rounding (f2bf), the k order and the epilogue are those of k_gemm,
 // so the results equal k_gemm's bit for bit.
-constexpr int kRowGroups = 16;  // row groups of 16 per wave
+constexpr int kRowGroups = 8;  // row groups of 16 per wave
+
+// A fragment of row m, k = k0 .. k0 + 7 (zeros past M or K)
+__device__ __forceinline__ void row_frag_load(const float* __restrict__ A, long M, int K, long m, int k0, bool vec,
+                                              float (&av)[8]) {
+    if (vec) {
+        float4 u = make_float4(0.f, 0.f, 0.f, 0.f), w = u;
+        if (m < M && k0 < K) {
+            const float4* p = reinterpret_cast<const float4*>(A + m * K + k0);
+            u = p[0];
+            w = p[1];
+        }
+        av[0] = u.x; av[1] = u.y; av[2] = u.z; av[3] = u.w;
+        av[4] = w.x; av[5] = w.y; av[6] = w.z; av[7] = w.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) av[j] = (m < M && k0 + j < K) ? A[m * K + k0 + j] : 0.f;
+    }
+}
 
 template <bool TRANSB>
 __global__ void __launch_bounds__(256) k_gemm_rows(const float* __restrict__ A, const float* __restrict__ B,
@@ -263,11 +281,21 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const float* __restrict__ A, 
     const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
     const int ntn = (N + 15) / 16, nks = (K + 31) / 32;
+    const long ngroups = (M + 15) / 16;
+    const long g0 = wave * kRowGroups, g1 = g0 + kRowGroups < ngroups ? g0 + kRowGroups : ngroups;
+    if (g0 >= g1) return;
+    const bool vec = (K % 8) == 0;
+    // the first group's rows are in flight while B becomes fragments
+    float av[2][8];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+        if (ks < nks) row_frag_load(A, M, K, g0 * 16 + r16, ks * 32 + kq, vec, av[ks]);
     bf16x8 fb[4][2];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
+            if (nt >= ntn || ks >= nks) continue;
             unsigned short t[8];
             const int n = nt * 16 + r16;
 #pragma unroll
@@ -279,41 +307,42 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const float* __restrict__ A, 
             }
             __builtin_memcpy(&fb[nt][ks], t, 16);
         }
-    const long ngroups = (M + 15) / 16;
-    const long g0 = wave * kRowGroups, g1 = g0 + kRowGroups < ngroups ? g0 + kRowGroups : ngroups;
-    const bool vec = (K % 8) == 0;
     for (long g = g0; g < g1; ++g) {
         const long m0 = g * 16;
+        // this group's C (beta != 0) loads before its MFMAs, so the epilogue
+        // does not wait a round trip per group
+        float cold[4][4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long mm = m0 + 4 * (lane >> 4) + r;
+                const int n = nt * 16 + r16;
+                cold[nt][r] = (beta != 0.f && nt < ntn && mm < M && n < N) ? C[mm * N + n] : 0.f;
+            }
+        bf16x8 fa[2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            unsigned short t[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = f2bf(av[ks][j]);
+            __builtin_memcpy(&fa[ks], t, 16);
+        }
+        // the next group's rows load while this group computes and stores
+        if (g + 1 < g1) {
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+                if (ks < nks) row_frag_load(A, M, K, m0 + 16 + r16, ks * 32 + kq, vec, av[ks]);
+        }
         f32x4 acc[4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const long m = m0 + r16;
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             if (ks >= nks) break;
-            const int k0 = ks * 32 + kq;
-            float av[8];
-            if (vec) {
-                float4 u = make_float4(0.f, 0.f, 0.f, 0.f), w = u;
-                if (m < M && k0 < K) {
-                    const float4* p = reinterpret_cast<const float4*>(A + m * K + k0);
-                    u = p[0];
-                    w = p[1];
-                }
-                av[0] = u.x; av[1] = u.y; av[2] = u.z; av[3] = u.w;
-                av[4] = w.x; av[5] = w.y; av[6] = w.z; av[7] = w.w;
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) av[j] = (m < M && k0 + j < K) ? A[m * K + k0 + j] : 0.f;
-            }
-            unsigned short t[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) t[j] = f2bf(av[j]);
-            bf16x8 fa;
-            __builtin_memcpy(&fa, t, 16);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
-                if (nt < ntn) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[nt][ks], acc[nt], 0, 0, 0);
+                if (nt < ntn) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks], fb[nt][ks], acc[nt], 0, 0, 0);
         }
         // D[m = 4 (l / 16) + r][n = l % 16]
 #pragma unroll
@@ -327,7 +356,7 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const float* __restrict__ A, 
                     const long off = mm * N + n;
                     float vr = alpha * acc[nt][r];
                     if (bias) vr += bias[n];
-                    if (beta != 0.f) vr += beta * C[off];
+                    if (beta != 0.f) vr += beta * cold[nt][r];
                     C[off] = vr;
                 }
             }
