@@ -287,7 +287,14 @@ int gemm_wgrad(gpe_fno* f, const float* A, const float* D, float* dW, long M, in
     a.scz = (long)K * N;
     if (M % chunk != 0) return ffail(f, "rows must be a multiple of 512");
     if ((size_t)nch * K * N > kKpartFloats / 2) return ffail(f, "split-K scratch too small");
-    if (gemm(f, a)) return 1;
+    if (K <= 64 && N <= 64 && ((uintptr_t)A % 16) == 0 && ((uintptr_t)D % 16) == 0) {
+        // the chunk kernel: coalesced slab loads, same partials as k_gemm
+        hipLaunchKernelGGL(gpe::gemm::k_gemm_wgrad, dim3(nch), dim3(256), 0, f->stream, A, D, f->kpart, (int)chunk, K,
+                           N);
+        FCHECK(hipGetLastError());
+    } else if (gemm(f, a)) {
+        return 1;
+    }
     return sum_chunks(f, f->kpart, nch, (long)K * N, dW);
 }
 
@@ -964,6 +971,47 @@ int gpe_gemm_rows_test(const float* A, const float* B, const float* bias, float*
         if (!rc) (void)hipMemcpy(C, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost);
     }
     for (float* p : {da, db, dc, dbias})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+// Weight-gradient partials entry used by the tests: P[z] = A_z^T D_z for
+// `chunks` chunks of 512 rows (A [chunks 512][K], D [chunks 512][N]) on the
+// chunk kernel (wg = 1) or the tile kernel (wg = 0).
+int gpe_gemm_wgrad_test(const float* A, const float* D, float* P, int chunks, int K, int N, int wg) {
+    if (chunks <= 0 || K <= 0 || K > 64 || N <= 0 || N > 64) return 1;
+    const long rows = 512;
+    float *da = nullptr, *dd = nullptr, *dp = nullptr;
+    int rc = 0;
+    rc |= hipMalloc(&da, (size_t)chunks * rows * K * 4) != hipSuccess;
+    rc |= hipMalloc(&dd, (size_t)chunks * rows * N * 4) != hipSuccess;
+    rc |= hipMalloc(&dp, (size_t)chunks * K * N * 4) != hipSuccess;
+    if (!rc) {
+        (void)hipMemcpy(da, A, (size_t)chunks * rows * K * 4, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dd, D, (size_t)chunks * rows * N * 4, hipMemcpyHostToDevice);
+        if (wg) {
+            hipLaunchKernelGGL(gpe::gemm::k_gemm_wgrad, dim3(chunks), dim3(256), 0, 0, da, dd, dp, (int)rows, K, N);
+        } else {
+            Args a = mkargs(K, N, (int)rows, chunks);
+            a.Ar = da;
+            a.sam = 1;
+            a.sak = K;
+            a.saz = rows * K;
+            a.Br = dd;
+            a.sbk = N;
+            a.sbn = 1;
+            a.sbz = rows * N;
+            a.Cr = dp;
+            a.scm = N;
+            a.scn = 1;
+            a.scz = (long)K * N;
+            dim3 grid((N + 63) / 64, (K + 63) / 64, chunks);
+            hipLaunchKernelGGL(gpe::gemm::k_gemm, grid, dim3(256), 0, 0, a);
+        }
+        rc = hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess;
+        if (!rc) (void)hipMemcpy(P, dp, (size_t)chunks * K * N * 4, hipMemcpyDeviceToHost);
+    }
+    for (float* p : {da, dd, dp})
         if (p) (void)hipFree(p);
     return rc;
 }

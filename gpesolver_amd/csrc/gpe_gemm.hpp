@@ -364,5 +364,103 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const float* __restrict__ A, 
     }
 }
 
+// Weight-gradient partials: for chunk z of `rows` rows, P_z[m][n] =
+// sum_r A[r][m] D[r][n] over the chunk's rows (A [R][K], D [R][N] row-major,
+// K, N <= 64), one block of 256 threads per chunk.  Each 32-row slab of A and
+// D is a contiguous block of memory: it is read with 16-byte loads (the next
+// slab's while this one computes), transposed into LDS as BF16 ([col][row],
+// so an MFMA fragment is one 16-byte LDS read), and the 16 x 16 output tiles
+// (m, n) are spread over the four waves.  Operand rounding, the k (row) order
+// and the MFMA are those of k_gemm on the same products, so the partials
+// equal k_gemm's bit for bit.
+constexpr int kWgSlab = 32;
+struct WgSmem {
+    unsigned short a[64][kWgSlab + 8];
+    unsigned short d[64][kWgSlab + 8];
+};
+
+__global__ void __launch_bounds__(256) k_gemm_wgrad(const float* __restrict__ A, const float* __restrict__ D,
+                                                    float* __restrict__ P, int rows, int K, int N) {
+    __shared__ WgSmem sm;
+    const long z = blockIdx.x;
+    const float* Az = A + z * (long)rows * K;
+    const float* Dz = D + z * (long)rows * N;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    const int mtn = (K + 15) / 16, ntn = (N + 15) / 16, ntiles = mtn * ntn;
+    const int na4 = kWgSlab * K / 4, nd4 = kWgSlab * N / 4;  // float4 per slab (32 K and 32 N are multiples of 4)
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // slab loads: 32 K + 32 N floats <= 4096, at most 4 float4 per thread
+    float4 pa[2], pd[2];
+    auto load_slab = [&](int s0) {
+        const float4* a4 = reinterpret_cast<const float4*>(Az + (long)s0 * K);
+        const float4* d4 = reinterpret_cast<const float4*>(Dz + (long)s0 * N);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * 256;
+            pa[q] = e < na4 ? a4[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+            pd[q] = e < nd4 ? d4[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    load_slab(0);
+    // zero padding past K / N (the slabs below write only columns < K, < N)
+    for (int e = threadIdx.x; e < 64 * kWgSlab; e += 256) {
+        const int c = e / kWgSlab, r = e % kWgSlab;
+        if (c >= K) sm.a[c][r] = 0;
+        if (c >= N) sm.d[c][r] = 0;
+    }
+    for (int s0 = 0; s0 < rows; s0 += kWgSlab) {
+        // the slab transposed into LDS
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * 256;
+            if (e < na4) {
+                const float v[4] = {pa[q].x, pa[q].y, pa[q].z, pa[q].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int f = 4 * e + j, r = f / K, c = f - r * K;
+                    sm.a[c][r] = f2bf(v[j]);
+                }
+            }
+            if (e < nd4) {
+                const float v[4] = {pd[q].x, pd[q].y, pd[q].z, pd[q].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int f = 4 * e + j, r = f / N, c = f - r * N;
+                    sm.d[c][r] = f2bf(v[j]);
+                }
+            }
+        }
+        __syncthreads();
+        if (s0 + kWgSlab < rows) load_slab(s0 + kWgSlab);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int tile = wave + 4 * t;
+            if (tile < ntiles) {
+                const int mt = tile / ntn, nt = tile - mt * ntn;
+                const bf16x8 fa = ld8(&sm.a[mt * 16 + r16][kq]);
+                const bf16x8 fb = ld8(&sm.d[nt * 16 + r16][kq]);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[t], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D[m = 4 (l / 16) + r][n = l % 16] of each tile
+    float* Pz = P + z * (long)K * N;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int tile = wave + 4 * t;
+        if (tile >= ntiles) break;
+        const int mt = tile / ntn, nt = tile - mt * ntn;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = mt * 16 + 4 * (lane >> 4) + r, n = nt * 16 + r16;
+            if (m < K && n < N) Pz[(long)m * N + n] = acc[t][r];
+        }
+    }
+}
+
 }  // namespace gemm
 }  // namespace gpe

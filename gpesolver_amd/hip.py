@@ -30,7 +30,7 @@ EXPORTS = (
     "gpe_fno_create", "gpe_fno_destroy", "gpe_fno_last_error", "gpe_fno_num_params", "gpe_fno_get_params",
     "gpe_fno_set_params", "gpe_fno_get_grads", "gpe_fno_forward", "gpe_fno_loss_grad", "gpe_fno_adam_step",
     "gpe_fno_set_data", "gpe_fno_loss_grad_idx",
-    "gpe_gemm_test", "gpe_gemm_rows_test",
+    "gpe_gemm_test", "gpe_gemm_rows_test", "gpe_gemm_wgrad_test",
 )
 # Every symbol include/gpesolver_host.h declares.
 HOST_EXPORTS = ("gpes_validate_params_json", "gpes_run", "gpes_write_files")

@@ -232,6 +232,11 @@ int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float
  * out.  Test entry point: the two kernels agree bit for bit. */
 int gpe_gemm_rows_test(const float *A, const float *B, const float *bias, float *C, long M, int N, int K, int transB,
                        float beta, int rows);
+/* Weight-gradient partials P[z] = A_z^T D_z of `chunks` chunks of 512 rows
+ * (A [chunks 512][K], D [chunks 512][N], K, N <= 64, host row-major) on the
+ * chunk kernel (wg = 1) or the tile kernel (wg = 0).  Test entry point: the
+ * two agree bit for bit. */
+int gpe_gemm_wgrad_test(const float *A, const float *D, float *P, int chunks, int K, int N, int wg);
 
 /* Autotuner (no reference counterpart).  Times the launch-shape variants of
  * the fused loops (one or two blocks per plane in the steady-state plane pass;

@@ -109,3 +109,28 @@ def test_row_gemm_equals_tile_gemm(M, N, K, beta, with_bias, transB):
     if bias is not None:
         ref += bias.astype(np.float64)
     assert np.abs(out[1] - ref).max() < 1e-4 * np.abs(ref).max() + 1e-5
+
+
+@pytest.mark.parametrize("K,N", [(32, 32), (6, 32), (32, 64), (64, 1), (64, 32), (1, 64)])
+def test_wgrad_chunk_kernel_equals_tile_gemm(K, N):
+    """The weight-gradient chunk kernel (coalesced 32-row slabs transposed in
+    LDS) gives the tile kernel's per-chunk partials A_z^T D_z bit for bit, and
+    they are the BF16 products with FP32 accumulation."""
+    L = hip.load()
+    fp = C.POINTER(C.c_float)
+    L.gpe_gemm_wgrad_test.argtypes = [fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.gpe_gemm_wgrad_test.restype = C.c_int
+    chunks = 3
+    rng = np.random.default_rng(K * 100 + N)
+    A = rng.standard_normal((chunks * 512, K)).astype(np.float32)
+    D = rng.standard_normal((chunks * 512, N)).astype(np.float32)
+    out = []
+    for wg in (0, 1):
+        P = np.full((chunks, K, N), np.nan, dtype=np.float32)
+        assert L.gpe_gemm_wgrad_test(A.ctypes.data_as(fp), D.ctypes.data_as(fp), P.ctypes.data_as(fp), chunks, K, N,
+                                     wg) == 0
+        out.append(P)
+    np.testing.assert_array_equal(out[1], out[0])
+    Ab, Db = bf16(A).astype(np.float64), bf16(D).astype(np.float64)
+    ref = np.stack([Ab[512 * z:512 * (z + 1)].T @ Db[512 * z:512 * (z + 1)] for z in range(chunks)])
+    assert np.abs(out[1] - ref).max() < 1e-4 * np.abs(ref).max()
