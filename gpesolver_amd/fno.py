@@ -196,3 +196,52 @@ def warm_state(pred: np.ndarray, shape, symmetric: bool = False) -> np.ndarray:
     psi = np.zeros(shape, dtype=np.complex128)
     psi[1:-1, 1:-1, 1:-1] = a[1:-1, 1:-1, 1:-1]
     return psi
+
+
+def time_training(steps: int = 10, batch: int = 8, seed: int = 0) -> dict:
+    """Wall time of the config-4 network at 64x64x16 (width 32, 4 layers, 16
+    modes, hidden 64) on random inputs: training steps from host arrays,
+    forward passes, and training steps from a device-resident set."""
+    import time
+
+    grid = (64, 64, 16)
+    npts = grid[0] * grid[1] * grid[2]
+    rng = np.random.default_rng(seed)
+    m = Fno(*grid, width=32, layers=4, hidden=64, modes=16, max_batch=batch, seed=1)
+    x = rng.standard_normal((batch, npts, CIN)).astype(np.float32)
+    t = rng.random((batch, npts)).astype(np.float32)
+    out = {}
+    try:
+        for _ in range(3):
+            m.loss_grad(x, t)
+            m.adam_step(1e-3)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.loss_grad(x, t)
+            m.adam_step(1e-3)
+        out["train_step_host_ms"] = (time.perf_counter() - t0) / steps * 1e3
+        t0 = time.perf_counter()
+        for _ in range(2 * steps):
+            m.forward(x)
+        out["forward_ms"] = (time.perf_counter() - t0) / (2 * steps) * 1e3
+        xs = rng.standard_normal((8 * batch, npts, CIN)).astype(np.float32)
+        ts = rng.random((8 * batch, npts)).astype(np.float32)
+        m.set_data(xs, ts)
+        for _ in range(3):
+            m.loss_grad_idx(np.arange(batch))
+            m.adam_step(1e-3)
+        t0 = time.perf_counter()
+        for s in range(steps):
+            m.loss_grad_idx(np.arange(batch * (s % 8), batch * (s % 8) + batch))
+            m.adam_step(1e-3)
+        out["train_step_device_ms"] = (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        m.close()
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    print(json.dumps(time_training(int(sys.argv[1]) if len(sys.argv) > 1 else 10)), flush=True)
