@@ -326,7 +326,7 @@ def main():
     ap.add_argument("--it-max-steps", type=int, default=20000, help="cap of the IT time-to-convergence leg")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg (trains the FNO, ~2 min)")
     ap.add_argument("--warm-train", type=int, default=2000, help="warm-start leg: generated training states")
-    ap.add_argument("--warm-epochs", type=int, default=40, help="warm-start leg: training epochs")
+    ap.add_argument("--warm-epochs", type=int, default=60, help="warm-start leg: training epochs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched IT sweep leg (~20 s)")
     ap.add_argument("--batch-n", type=int, default=64, help="batched IT sweep leg: instances")
     ap.add_argument("--rt-only", action="store_true",

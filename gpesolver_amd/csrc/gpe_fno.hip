@@ -881,6 +881,16 @@ int gpe_fno_loss_grad(gpe_fno* f, const float* x, const float* target, int B, do
     return fno_backward(f, B);
 }
 
+float* gpe_fno_input_dev(gpe_fno* f) { return f ? f->xin : nullptr; }
+const float* gpe_fno_output_dev(gpe_fno* f) { return f ? f->y : nullptr; }
+
+int gpe_fno_forward_dev(gpe_fno* f, int B) {
+    if (!f || B <= 0 || B > f->maxb) return ffail(f, "bad batch size");
+    if (fno_forward(f, B)) return 1;
+    FCHECK(hipStreamSynchronize(f->stream));
+    return 0;
+}
+
 int gpe_fno_set_data(gpe_fno* f, const float* x, const float* target, long n) {
     if (!f || n <= 0 || !x || !target) return ffail(f, "bad training set");
     FCHECK(hipStreamSynchronize(f->stream));  // no launch may still read the old set

@@ -4564,6 +4564,89 @@ static int do_init_psi(gpe_handle* h, const InitArgs& a) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Warm start from a network prediction on the device (no reference
+// counterpart; the state enters as load_buffer does, src/solver/solver.cpp:
+// 118-125).
+
+// Network input of one instance, [P][cin]: channel 0 the potential over
+// v_scale, channels 1 .. cin-1 the instance's constants.
+struct WarmConsts {
+    float c[8];
+};
+
+template <typename T>
+__global__ void k_warm_features(const T* __restrict__ V, size_t n, int cin, double v_scale, WarmConsts c,
+                                float* __restrict__ x) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        float* xe = x + e * cin;
+        xe[0] = (float)((double)V[e] / v_scale);
+        for (int q = 1; q < cin; ++q) xe[q] = c.c[q - 1];
+    }
+}
+
+// psi0 from a predicted amplitude: interior points max(a, 0), the
+// boundary zero; `mirror` first averages a with its mirror images i -> n - i
+// along x, then y, then z over the interior pairs 2 <= i <= n - 2 (indices 0,
+// 1 and n - 1 keep their value), in FP64 and in that order, so the state
+// equals the host's warm state fed to load_buffer bit for bit.  Stored as
+// (T)(psi0 sqrt(dV)) in the state and in psii, as load_buffer stores.
+template <typename T>
+__global__ void k_warm_state(const float* __restrict__ amp, int nx, int ny, int nz, int mirror, double sqdv,
+                             cplx<T>* __restrict__ store, cplx<T>* __restrict__ psii) {
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % nz), j = (int)((e / nz) % ny), i = (int)(e / ((size_t)ny * nz));
+        double w = 0.0;
+        if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
+            auto A = [&](int ii, int jj, int kk) { return (double)amp[((size_t)ii * ny + jj) * nz + kk]; };
+            if (!mirror) {
+                w = A(i, j, k);
+            } else {
+                const bool mx = i >= 2 && i <= nx - 2, my = j >= 2 && j <= ny - 2, mz = k >= 2 && k <= nz - 2;
+                auto tx = [&](int jj, int kk) {
+                    const double a = A(i, jj, kk);
+                    return mx ? 0.5 * (a + A(nx - i, jj, kk)) : a;
+                };
+                auto ty = [&](int kk) {
+                    const double t = tx(j, kk);
+                    return my ? 0.5 * (t + tx(ny - j, kk)) : t;
+                };
+                const double u = ty(k);
+                w = mz ? 0.5 * (u + ty(nz - k)) : u;
+            }
+            w = isnan(w) ? w : (w > 0.0 ? w : 0.0);
+        }
+        const T v = (T)(w * sqdv);
+        store[e] = gpe::mk(v, (T)(0.0 * sqdv));
+        psii[e] = store[e];
+    }
+}
+
+template <typename T>
+static int do_warm_features(gpe_handle* h, float* x, int cin, double v_scale, const WarmConsts& c) {
+    hipLaunchKernelGGL((k_warm_features<T>), dim3(1024), dim3(256), 0, h->stream, (const T*)h->V, h->N, cin, v_scale,
+                       c, x);
+    GPE_CHECK(hipGetLastError());
+    GPE_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+template <typename T>
+static int do_warm_state(gpe_handle* h, const float* amp, int mirror, double* norm_out) {
+    if (flush_rt<T>(h)) return 1;
+    hipLaunchKernelGGL((k_warm_state<T>), dim3(1024), dim3(256), 0, h->stream, amp, h->p.nx, h->p.ny, h->p.nz, mirror,
+                       h->sqdv, (cplx<T>*)h->store[h->cur], (cplx<T>*)h->psii);
+    GPE_CHECK(hipGetLastError());
+    h->norm_pending = false;
+    h->rt_pending = false;  // the loaded state replaces a pending RT step
+    // calc_norm + normalize, as load_buffer (solver.cpp:118-125)
+    double xn = 0.0;
+    if (do_calc_norm<T>(h, &xn)) return 1;
+    if (norm_out) *norm_out = xn;
+    return do_normalize<T>(h);
+}
+
 #define DISPATCH(h, fn, ...) ((h)->prec == GPE_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
 extern "C" {
@@ -4723,6 +4806,19 @@ int gpe_load_buffer(gpe_handle* h, const double* psi) {
     // calc_norm + normalize (solver.cpp:118-125)
     if (DISPATCH(h, do_calc_norm, h, nullptr)) return 1;
     return DISPATCH(h, do_normalize, h);
+}
+
+int gpe_warm_features(gpe_handle* h, float* x_dev, int cin, double v_scale, const float* consts) {
+    if (!h || !x_dev || !consts) return fail(h, "null argument");
+    if (cin < 1 || cin > 9) return fail(h, "gpe_warm_features: 1 <= cin <= 9");
+    WarmConsts c{};
+    for (int q = 0; q + 1 < cin; ++q) c.c[q] = consts[q];
+    return DISPATCH(h, do_warm_features, h, x_dev, cin, v_scale, c);
+}
+
+int gpe_load_amplitude(gpe_handle* h, const float* amp_dev, int mirror, double* norm_out) {
+    if (!h || !amp_dev) return fail(h, "null argument");
+    return DISPATCH(h, do_warm_state, h, amp_dev, mirror, norm_out);
 }
 
 int gpe_load_pote(gpe_handle* h, const double* V) {

@@ -51,6 +51,12 @@ def _bind():
     L.gpe_fno_loss_grad.restype = ip
     L.gpe_fno_adam_step.argtypes = [vp, C.c_float]
     L.gpe_fno_adam_step.restype = ip
+    L.gpe_fno_input_dev.argtypes = [vp]
+    L.gpe_fno_input_dev.restype = vp
+    L.gpe_fno_output_dev.argtypes = [vp]
+    L.gpe_fno_output_dev.restype = vp
+    L.gpe_fno_forward_dev.argtypes = [vp, ip]
+    L.gpe_fno_forward_dev.restype = ip
     L.gpe_fno_set_data.argtypes = [vp, fp, fp, C.c_long]
     L.gpe_fno_set_data.restype = ip
     L.gpe_fno_loss_grad_idx.argtypes = [vp, C.POINTER(C.c_int), ip, dp]
@@ -125,6 +131,19 @@ class Fno:
         self._check(self.L.gpe_fno_loss_grad(self.f, _fp(x), _fp(t), x.shape[0], C.byref(loss)))
         return loss.value
 
+    def input_ptr(self, slot: int = 0) -> int:
+        """Device address of batch slot `slot` of the network input, [P][cin]."""
+        return self.L.gpe_fno_input_dev(self.f) + slot * self.P * self.cfg.cin * 4
+
+    def output_ptr(self, slot: int = 0) -> int:
+        """Device address of the output of batch slot `slot`, [P]."""
+        return self.L.gpe_fno_output_dev(self.f) + slot * self.P * 4
+
+    def forward_device(self, batch: int):
+        """Forward pass of the first `batch` inputs already on the device;
+        the output stays there (output_ptr)."""
+        self._check(self.L.gpe_fno_forward_dev(self.f, batch))
+
     def set_data(self, x: np.ndarray, t: np.ndarray):
         """Keep the training set [n, P, cin] / [n, P] on the device."""
         x = np.ascontiguousarray(x, dtype=np.float32)
@@ -144,14 +163,19 @@ class Fno:
         self._check(self.L.gpe_fno_adam_step(self.f, lr))
 
 
+def feature_consts(p: P.PhysicalParameters, pot: str) -> np.ndarray:
+    """Input channels 1 .. CIN-1 of an instance (constant over the grid)."""
+    return np.array([(p.edd - 1.4) / 0.2, (math.log10(p.n_atoms) - 4.35) / 0.5,
+                     (P.freq_au_to_Hz(p.omega_z) - 120.0) / 20.0, 1.0 if pot.startswith("MEXICAN") else 0.0,
+                     P.len_au_to_nm(p.dd) / 1500.0], dtype=np.float32)
+
+
 def features(p: P.PhysicalParameters, V: np.ndarray, pot: str) -> np.ndarray:
     """Per-point input channels [P, CIN] for one instance (see module doc)."""
     n = V.size
     x = np.empty((n, CIN), dtype=np.float32)
     x[:, 0] = (V.reshape(-1) / V0).astype(np.float32)
-    x[:, 1:] = np.array([(p.edd - 1.4) / 0.2, (math.log10(p.n_atoms) - 4.35) / 0.5,
-                         (P.freq_au_to_Hz(p.omega_z) - 120.0) / 20.0, 1.0 if pot.startswith("MEXICAN") else 0.0,
-                         P.len_au_to_nm(p.dd) / 1500.0], dtype=np.float32)
+    x[:, 1:] = feature_consts(p, pot)
     return x
 
 

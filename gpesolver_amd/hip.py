@@ -29,7 +29,8 @@ EXPORTS = (
     "gpe_batch_set_fallback", "gpe_batch_fallback_taken", "gpe_batch_autotune",
     "gpe_fno_create", "gpe_fno_destroy", "gpe_fno_last_error", "gpe_fno_num_params", "gpe_fno_get_params",
     "gpe_fno_set_params", "gpe_fno_get_grads", "gpe_fno_forward", "gpe_fno_loss_grad", "gpe_fno_adam_step",
-    "gpe_fno_set_data", "gpe_fno_loss_grad_idx",
+    "gpe_fno_set_data", "gpe_fno_loss_grad_idx", "gpe_fno_input_dev", "gpe_fno_output_dev", "gpe_fno_forward_dev",
+    "gpe_warm_features", "gpe_load_amplitude",
     "gpe_gemm_test", "gpe_gemm_rows_test", "gpe_gemm_wgrad_test",
 )
 # Every symbol include/gpesolver_host.h declares.
@@ -87,6 +88,10 @@ def load(path: str = LIB_PATH):
     L.gpe_last_error.argtypes = [vp]
     L.gpe_last_error.restype = C.c_char_p
     L.gpe_version.restype = C.c_char_p
+    L.gpe_warm_features.argtypes = [vp, vp, ip, C.c_double, C.POINTER(C.c_float)]
+    L.gpe_warm_features.restype = ip
+    L.gpe_load_amplitude.argtypes = [vp, vp, ip, dp]
+    L.gpe_load_amplitude.restype = ip
     L.gpe_init_potential.argtypes = [vp, ip, C.POINTER(GpeTrap)]
     L.gpe_init_potential.restype = ip
     L.gpe_init_wavefunction.argtypes = [vp, ip, C.POINTER(GpeTrap), ip, ip, ip, ip, C.c_double, C.c_double,
@@ -256,6 +261,20 @@ class Device:
         """psi_0 of `strategy` on the device, then load_buffer's norm and normalize."""
         self._check(self.L.gpe_init_wavefunction(self.h, INIT_KEYS[strategy], C.byref(self.trap_of(p)), n_gauss_max,
                                                  drops[0], drops[1], drops[2], sigma_cells, phase_noise, seed))
+
+    def warm_features(self, x_dev: int, cin: int, v_scale: float, consts):
+        """Network input of this instance, [P][cin] at device address x_dev
+        (gpe_warm_features)."""
+        c = (C.c_float * 8)(*[float(v) for v in consts])
+        self._check(self.L.gpe_warm_features(self.h, C.c_void_p(x_dev), cin, v_scale, c))
+
+    def load_amplitude(self, amp_dev: int, mirror: bool) -> float:
+        """psi0 from the predicted amplitude at device address amp_dev, loaded
+        and normalised like load_buffer (gpe_load_amplitude); returns the norm
+        before normalising."""
+        n = C.c_double()
+        self._check(self.L.gpe_load_amplitude(self.h, C.c_void_p(amp_dev), 1 if mirror else 0, C.byref(n)))
+        return n.value
 
     def export_pote(self) -> np.ndarray:
         V = np.empty(self.shape, dtype=np.float64)

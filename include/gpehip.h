@@ -222,6 +222,26 @@ int gpe_fno_adam_step(gpe_fno *f, float lr);
  * does from host arrays. */
 int gpe_fno_set_data(gpe_fno *f, const float *x, const float *target, long n);
 int gpe_fno_loss_grad_idx(gpe_fno *f, const int *idx, int batch, double *loss);
+/* The network's input and output buffers on the device ([max_batch][P][cin]
+ * and [max_batch][P], FP32; valid until gpe_fno_destroy) and a forward pass
+ * of the first `batch` inputs that leaves its output there: with
+ * gpe_warm_features and gpe_load_amplitude the warm start never leaves the
+ * device. */
+float *gpe_fno_input_dev(gpe_fno *f);
+const float *gpe_fno_output_dev(gpe_fno *f);
+int gpe_fno_forward_dev(gpe_fno *f, int batch);
+/* Warm start on the device (no reference counterpart).  gpe_warm_features
+ * writes the network input of instance h, [P][cin] at x_dev (a device
+ * pointer, e.g. gpe_fno_input_dev + slot P cin): channel 0 = V / v_scale from
+ * the instance's potential, channels 1 .. cin-1 = consts[0 .. cin-2].
+ * gpe_load_amplitude loads psi0 = max(a, 0) on the interior, zero on the
+ * boundary, from the predicted amplitude a at amp_dev ([P], device; with
+ * mirror, a is first averaged with its mirror images along x, y, z over the
+ * interior pairs), then normalises as gpe_load_buffer does
+ * (src/solver/solver.cpp:118-125); *norm_out receives dV sum |psi0|^2 before
+ * normalising, NaN or 0 for a prediction the caller must replace. */
+int gpe_warm_features(gpe_handle *h, float *x_dev, int cin, double v_scale, const float *consts);
+int gpe_load_amplitude(gpe_handle *h, const float *amp_dev, int mirror, double *norm_out);
 /* C = op(A) op(B) on the MFMA GEMM (row-major host matrices; imaginary parts
  * may be NULL; Ci NULL keeps only the real part).  Test entry point. */
 int gpe_gemm_test(const float *Ar, const float *Ai, const float *Br, const float *Bi, float *Cr, float *Ci, int M,

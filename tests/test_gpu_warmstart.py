@@ -79,3 +79,55 @@ def test_fallback_on_energy_rise():
         b.close()
     print("steps", steps, "E/N", e, "taken", taken, flush=True)
     assert taken.tolist() == [True, False]
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp32", "fp64"])
+def test_device_warm_start_equals_host_path(prec):
+    """The warm start on the device (gpe_warm_features from the instance's
+    potential, gpe_fno_forward_dev, gpe_load_amplitude with and without the
+    mirror average) gives the host path's state bit for bit: features of the
+    exported potential, forward pass from host arrays, warm_state fed to
+    gpe_load_buffer.  The norm returned is the loaded state's dV sum |psi|^2."""
+    from gpesolver_amd import hip
+
+    precision = (hip.GPE_FP32, hip.GPE_FP64)[prec]
+    rng = np.random.default_rng(21)
+    model = F.Fno(*W.GRID, width=8, layers=1, hidden=8, modes=4, max_batch=2, seed=5)
+    insts = W.sample_instances(rng, 2)
+    w0 = model.get_params()
+    try:
+        for d in insts:
+            p, V, _ = W.build(d)
+            # shift the output bias (the last parameter) so that about three
+            # quarters of this instance's prediction is positive: the clip to
+            # zero and the mirror average both act
+            model.set_params(w0)
+            y0 = model.forward(np.stack([F.features(p, V, d["pot"])] * 2))[0]
+            w = w0.copy()
+            w[-1] -= np.float32(np.quantile(y0, 0.25))
+            model.set_params(w)
+            dev = hip.Device(p.to_gpe(), precision)
+            ref = hip.Device(p.to_gpe(), precision)
+            try:
+                dev.init_potential(d["pot"], p)
+                dev.warm_features(model.input_ptr(1), F.CIN, F.V0, F.feature_consts(p, d["pot"]))
+                model.forward_device(2)
+                x = F.features(p, dev.export_pote(), d["pot"])
+                for mirror in (True, False):
+                    norm = dev.load_amplitude(model.output_ptr(1), mirror)
+                    got = dev.export_psi()
+                    y = model.forward(np.stack([x, x]))[1]
+                    assert (y > 0).mean() > 0.3 and (y < 0).any(), (float(np.min(y)), float(np.max(y)))
+                    psi = F.warm_state(y, W.GRID, symmetric=mirror)
+                    ref.load_buffer(psi)
+                    np.testing.assert_array_equal(got, ref.export_psi())
+                    dv = p.dx * p.dy * p.dz
+                    assert abs(norm - dv * np.sum(np.abs(psi) ** 2)) < 1e-5 * norm
+                    # the device input was overwritten by the host forward pass: rebuild it
+                    dev.warm_features(model.input_ptr(1), F.CIN, F.V0, F.feature_consts(p, d["pot"]))
+                    model.forward_device(2)
+            finally:
+                dev.close()
+                ref.close()
+    finally:
+        model.close()
