@@ -4087,6 +4087,14 @@ static int line_slot(int kind, int nx) {
 #undef GPE_LS
     return r;
 }
+// GPE_BATCH_SINGLE=0: a lone active instance keeps the batch's launches.
+static bool batch_single_on() {
+    static const bool on = [] {
+        const char* e = getenv("GPE_BATCH_SINGLE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
 // GPE_BATCH_MULTI=0: one launch per grid shape and pass.
 static bool batch_multi_on() {
     static const bool on = [] {
@@ -4434,11 +4442,14 @@ static int batch_it(gpe_batch* b, long n) {
     if (batch_line<T>(b) || batch_plane<T>(b, 8 + b->cur * 2 + b->pcur)) return 1;
     b->cur = 1 - b->cur;
     b->pcur = 1 - b->pcur;
-    for (gpe_handle* h : b->inst) {
+    // only the stepped instances move: a dropped one keeps the buffer its
+    // state was left in (n may be odd)
+    for (int q : b->active) {
+        gpe_handle* h = b->inst[q];
         h->cur = b->cur;
         h->pcur = b->pcur;
+        h->norm_pending = true;
     }
-    for (int q : b->active) b->inst[q]->norm_pending = true;
     return 0;
 }
 
@@ -4474,7 +4485,21 @@ static int batch_it_until(gpe_batch* b, long max_steps, long check_every, double
     if (batch_setup<T>(b)) return 1;
     while (!b->active.empty() && steps < max_steps) {
         const long n = std::min(check_every, max_steps - steps);
-        if (batch_it<T>(b, n)) return 1;
+        if (b->active.size() == 1 && batch_single_on()) {
+            // A lone instance (the tail of a batch) runs its handle's own
+            // fused loop (gpe_imag_time_iters): its tuned plane pass, two
+            // blocks per plane, and graph replay, on the batch's stream and
+            // state parity.  The same steps; in FP32 the last bits may round
+            // as the single-instance path rounds them.
+            gpe_handle* h = b->inst[b->active[0]];
+            h->cur = b->cur;
+            h->pcur = b->pcur;
+            if (do_it_iters<T>(h, n)) return bfail(b, h->err);
+            b->cur = h->cur;
+            b->pcur = h->pcur;
+        } else if (batch_it<T>(b, n)) {
+            return 1;
+        }
         steps += n;
         if (batch_check<T>(b, b->h_eout)) return 1;
         std::vector<int> keep;

@@ -173,8 +173,10 @@ gpe_handle *gpe_batch_instance(gpe_batch *b, int i);
 const char *gpe_batch_last_error(const gpe_batch *b);
 /* Imaginary time on every instance until |E/N(k) - E/N(k - check_every)| <
  * tol |E/N(k)| (energies checked every check_every steps) or max_steps; a
- * converged instance leaves the launch set.  steps_out / energy_per_atom_out
- * (length n, may be NULL) receive each instance's step count and final E/N. */
+ * converged instance leaves the launch set, and the last instance left runs
+ * its handle's own fused loop (gpe_imag_time_iters; GPE_BATCH_SINGLE=0 keeps
+ * the batch's launches).  steps_out / energy_per_atom_out (length n, may be
+ * NULL) receive each instance's step count and final E/N. */
 int gpe_batch_it_until(gpe_batch *b, long max_steps, long check_every, double tol, long *steps_out,
                        double *energy_per_atom_out);
 /* Fallback of a warm-started instance (no reference counterpart; the

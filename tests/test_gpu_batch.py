@@ -131,3 +131,31 @@ def test_batch_groups_share_shapes_and_generic_kernels(monkeypatch, generic):
         assert np.array_equal(b.instances[q].export_psi(), s.export_psi()), q
         s.close()
     b.close()
+
+
+def test_dropped_instances_keep_their_state_with_odd_check_interval():
+    """A check interval that is odd flips the state buffer on every check:
+    an instance dropped on convergence keeps the buffer its state is in while
+    the others step on, and the last one left runs its handle's own loop.
+    Every instance stops where its single handle stops, with its state (FP64,
+    where the batched and single kernels agree to rounding)."""
+    conv_cases = [((16, 16, 8), 1.4, 1e4), ((64, 64, 16), 1.5, 2e4), ((32, 24, 12), 1.3, 8e3)]
+    setups = [setup_instance(*c) for c in conv_cases]
+    b = hip.Batch([s[0].to_gpe() for s in setups], hip.GPE_FP64)
+    for d, (p, V, psi) in zip(b.instances, setups):
+        d.load_pote(V)
+        d.load_buffer(psi)
+    steps, e = b.it_until(3000, check_every=75, tol=1e-6)
+    print("odd interval", steps.tolist(), flush=True)
+    srt = sorted(steps.tolist())
+    assert srt[0] < srt[-1] and srt[-2] < srt[-1]  # dropped early, then a lone instance
+    for q, (p, V, psi) in enumerate(setups):
+        s = hip.Device(p.to_gpe(), hip.GPE_FP64)
+        s.load_pote(V)
+        s.load_buffer(psi)
+        st, es = s.it_until(3000, check_every=75, tol=1e-6)
+        assert st == int(steps[q]), q
+        assert abs(es - e[q]) <= 1e-12 * abs(es), q
+        assert rel(b.instances[q].export_psi(), s.export_psi()) < 1e-12, q
+        s.close()
+    b.close()
