@@ -1043,7 +1043,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 if (!(mode & M_NO_PSI_OUT)) store_wt(&a.out[plane_off + p], r);
                 const T n2 = r.x * r.x + r.y * r.y;
                 rho_r[q] = n2;
-                nacc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+                if (mode & M_NORM_OUT) nacc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
             }
         }
         __syncthreads();
@@ -1113,7 +1113,7 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                 // which leaves it unchanged (the split pass's Poisson-start
                 // block never applies the phase, and both agree bit for bit)
                 rho_r[q] = nrm;
-                nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
+                if (mode & M_NORM_OUT) nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
             }
         }
         __syncthreads();
@@ -1486,7 +1486,7 @@ __global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
                 a.out[e] = o;
                 cv[q] = o;
                 rho_r[q] = o.x * o.x + o.y * o.y;
-                nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+                if (mode & M_NORM_OUT) nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
             }
         }
     } else if (have_psi) {
@@ -1528,7 +1528,7 @@ __global__ void __launch_bounds__(NT) k_pencil_r(PlaneArgs<T> a) {
                 }
                 cv[q] = c;
                 rho_r[q] = nrm;
-                nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
+                if (mode & M_NORM_OUT) nacc += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
                 if (mode & M_OUT) a.out[e] = c;
             }
         }
@@ -1742,7 +1742,7 @@ __global__ void __launch_bounds__(256) k_pencil_rz(PlaneArgs<T> a) {
             store_wt(&a.out[e], o);
             c[q] = o;
             rho[q] = o.x * o.x + o.y * o.y;
-            nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+            if (mode & M_NORM_OUT) nacc += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
         }
     } else {
         const float tv = (float)(a.hdt * a.cv), tphi = (float)(a.hdt * a.cv * a.cdd), tg = (float)(a.hdt * pg),
@@ -1771,7 +1771,7 @@ __global__ void __launch_bounds__(256) k_pencil_rz(PlaneArgs<T> a) {
             }
             c[q] = cc;
             rho[q] = nrm;  // the density before the phase, which leaves it unchanged
-            nacc += (double)cc.x * (double)cc.x + (double)cc.y * (double)cc.y;
+            if (mode & M_NORM_OUT) nacc += (double)cc.x * (double)cc.x + (double)cc.y * (double)cc.y;
             if (mode & M_OUT) store_wt(&a.out[row_off + k], cc);
         }
     }
