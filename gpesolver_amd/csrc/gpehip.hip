@@ -762,22 +762,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
     constexpr int TQ = kStage ? 2 : 1;  // twiddles per thread in that round trip
     cplx<T> tq[TQ], kq[4];
     if constexpr (kStage) {
-        if (mode & M_PINV) {
-            if constexpr (kRegY2) {
-                constexpr int B = Y2Reg<Pol::kNY, Pol::kNZ>::B;
-                if (threadIdx.x < Y2Reg<Pol::kNY, Pol::kNZ>::NTH) {
-                    const int c = threadIdx.x % nz, b = threadIdx.x / nz;
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) sy[q] = a.s1[((size_t)(B * q + b) * nx + i) * nz + c];
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < R2; ++q) {
-                    const int e = threadIdx.x + q * NT;
-                    if (e < ny2 * nz) st[q] = a.s1[((size_t)(e / nz) * nx + i) * nz + e % nz];
-                }
-            }
-        }
+        // psi and V first: the phase terms without Phi are formed from them
+        // while the S1 loads are still in flight
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int p = threadIdx.x + q * NT;
@@ -793,6 +779,22 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                     }
                 }
                 if (need_v) vpre[q] = a.V[plane_off + p];
+            }
+        }
+        if (mode & M_PINV) {
+            if constexpr (kRegY2) {
+                constexpr int B = Y2Reg<Pol::kNY, Pol::kNZ>::B;
+                if (threadIdx.x < Y2Reg<Pol::kNY, Pol::kNZ>::NTH) {
+                    const int c = threadIdx.x % nz, b = threadIdx.x / nz;
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) sy[q] = a.s1[((size_t)(B * q + b) * nx + i) * nz + c];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < R2; ++q) {
+                    const int e = threadIdx.x + q * NT;
+                    if (e < ny2 * nz) st[q] = a.s1[((size_t)(e / nz) * nx + i) * nz + e % nz];
+                }
             }
         }
         static_assert(Pol::kNY <= NT && Pol::kNZ <= NT, "one kinetic table element per thread");
@@ -832,6 +834,23 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         const bool full = (mode & M_PHASE_FULL) != 0;
         pg = full ? c0 + c2 : c0;
         pgm = full ? c1 + c3 : c1;
+    }
+
+    // FP32 phase argument without its Phi term, tv V + tg |psi|^2 +
+    // tgm |psi|^3, from the staged psi and V (the plane as the phase reads it:
+    // Tk, or an input without the norm scale); the phase adds tphi Phi
+    const bool th0_on = kStage && sizeof(T) == 4 && (mode & (M_PHASE_HALF | M_PHASE_FULL)) &&
+                        ((mode & M_IN_TK) || !(mode & M_NORM_IN));
+    float th0[PRE];
+    if constexpr (kStage && sizeof(T) == 4) {
+        if (th0_on) {
+            const float tv = (float)(a.hdt * a.cv), tg = (float)(a.hdt * pg), tgm = (float)(a.hdt * pgm);
+#pragma unroll
+            for (int q = 0; q < PRE; ++q) {
+                const float nrm = (float)pre[q].x * (float)pre[q].x + (float)pre[q].y * (float)pre[q].y;
+                th0[q] = tv * (float)vpre[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
+            }
+        }
     }
 
     if constexpr (kStage) {
@@ -1091,7 +1110,8 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                         bool done = false;
                         if constexpr (sizeof(T) == 4) {
                             const float vv = kStage ? vpre[q < PRE ? q : 0] : a.V[plane_off + p];
-                            const float t32 = tv * vv + tphi * phi_r[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
+                            const float t32 = th0_on ? th0[q < PRE ? q : 0] + tphi * phi_r[q]
+                                                     : tv * vv + tphi * phi_r[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
                             if (fabsf(t32) <= 0.0625f) {
                                 c = gpe::phase_rot_small(c, -t32);
                                 done = true;
