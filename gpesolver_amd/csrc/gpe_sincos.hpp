@@ -77,14 +77,56 @@ __device__ __forceinline__ cplx<float> phase_rot_small(cplx<float> c, float r) {
     return o;
 }
 
-// FP64: the FP64 sincos of the reduced phase.
-__device__ __forceinline__ cplx<double> phase_rot(cplx<double> c, double theta) {
+// c * exp(i theta) in FP64 with the library sincos of theta reduced by 2 pi:
+// the run-time-size plane kernel, whose registers are already at the limit,
+// keeps it (the inline polynomials below make it spill).
+__device__ __forceinline__ cplx<double> phase_rot_lib(cplx<double> c, double theta) {
     double s, co;
     theta -= 6.283185307179586477 * rint(theta * 0.15915494309189533577);
     sincos(theta, &s, &co);
     cplx<double> o;
     o.x = c.x * co - c.y * s;
     o.y = c.x * s + c.y * co;
+    return o;
+}
+
+// c * exp(i theta) in FP64.  theta is reduced by the nearest multiple of
+// pi/2 with Cody and Waite's two-constant method (r = (theta - q P1) - q P2,
+// P1 = pi/2 rounded to FP64, P2 = pi/2 - P1, each product inside an FMA), so
+// |r| <= pi/4 to ~1e-16 for the phases of a step.  sin r and cos r are their
+// Taylor series through r^15 and r^16 (coefficients (-1)^n / (2n+1)! and
+// (-1)^n / (2n)!, Horner in u = r^2); the first omitted terms, r^17 / 17! and
+// r^18 / 18!, are below 5e-17 on |r| <= pi/4.  The quadrant q turns the value
+// exactly by q quarter turns.
+__device__ __forceinline__ cplx<double> phase_rot(cplx<double> c, double theta) {
+    const double q = rint(theta * 0.63661977236758134308);  // 2/pi
+    double r = fma(-q, 1.5707963267948966, theta);
+    r = fma(-q, 6.123233995736766e-17, r);
+    const double u = r * r;
+    double ps = -1.0 / 1307674368000.0;
+    ps = fma(ps, u, 1.0 / 6227020800.0);
+    ps = fma(ps, u, -1.0 / 39916800.0);
+    ps = fma(ps, u, 1.0 / 362880.0);
+    ps = fma(ps, u, -1.0 / 5040.0);
+    ps = fma(ps, u, 1.0 / 120.0);
+    ps = fma(ps, u, -1.0 / 6.0);
+    const double sr = fma(r * u, ps, r);
+    double pc = 1.0 / 20922789888000.0;
+    pc = fma(pc, u, -1.0 / 87178291200.0);
+    pc = fma(pc, u, 1.0 / 479001600.0);
+    pc = fma(pc, u, -1.0 / 3628800.0);
+    pc = fma(pc, u, 1.0 / 40320.0);
+    pc = fma(pc, u, -1.0 / 720.0);
+    pc = fma(pc, u, 1.0 / 24.0);
+    pc = fma(pc, u, -0.5);
+    const double cr = fma(u, pc, 1.0);
+    const int k = ((int)(long long)q) & 3;
+    // b = c * i^k
+    const double bx = (k == 0) ? c.x : (k == 1) ? -c.y : (k == 2) ? -c.x : c.y;
+    const double by = (k == 0) ? c.y : (k == 1) ? c.x : (k == 2) ? -c.y : -c.x;
+    cplx<double> o;
+    o.x = fma(bx, cr, -(by * sr));
+    o.y = fma(bx, sr, by * cr);
     return o;
 }
 

@@ -836,19 +836,29 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
         pgm = full ? c1 + c3 : c1;
     }
 
-    // FP32 phase argument without its Phi term, tv V + tg |psi|^2 +
-    // tgm |psi|^3, from the staged psi and V (the plane as the phase reads it:
-    // Tk, or an input without the norm scale); the phase adds tphi Phi
-    const bool th0_on = kStage && sizeof(T) == 4 && (mode & (M_PHASE_HALF | M_PHASE_FULL)) &&
+    // The phase argument without its Phi term, from the staged psi and V
+    // (the plane as the phase reads it: Tk, or an input without the norm
+    // scale), formed while the S1 loads are in flight.  FP32: tv V + tg |psi|^2
+    // + tgm |psi|^3 of the small-phase path, which adds tphi Phi.  FP64:
+    // cv V + pg |psi|^2 + pgm |psi|^3, to which the phase adds cv cdd Phi.
+    const bool th0_on = kStage && (mode & (M_PHASE_HALF | M_PHASE_FULL)) &&
                         ((mode & M_IN_TK) || !(mode & M_NORM_IN));
-    float th0[PRE];
-    if constexpr (kStage && sizeof(T) == 4) {
+    T th0[PRE];
+    if constexpr (kStage) {
         if (th0_on) {
-            const float tv = (float)(a.hdt * a.cv), tg = (float)(a.hdt * pg), tgm = (float)(a.hdt * pgm);
+            if constexpr (sizeof(T) == 4) {
+                const float tv = (float)(a.hdt * a.cv), tg = (float)(a.hdt * pg), tgm = (float)(a.hdt * pgm);
 #pragma unroll
-            for (int q = 0; q < PRE; ++q) {
-                const float nrm = (float)pre[q].x * (float)pre[q].x + (float)pre[q].y * (float)pre[q].y;
-                th0[q] = tv * (float)vpre[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
+                for (int q = 0; q < PRE; ++q) {
+                    const float nrm = pre[q].x * pre[q].x + pre[q].y * pre[q].y;
+                    th0[q] = tv * vpre[q] + tg * nrm + tgm * (nrm * sqrtf(nrm));
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < PRE; ++q) {
+                    const double dn = dens64(pre[q]);
+                    th0[q] = a.cv * vpre[q] + pg * dn + pgm * (dn * sqrt(dn));
+                }
             }
         }
     }
@@ -1121,10 +1131,19 @@ __device__ __forceinline__ void plane_body(const PlaneArgs<T>& a, const int i, c
                             // phase argument in FP64, reduced in FP64 before the
                             // sin / cos of the bulk precision, so a large phase (dense
                             // states, long steps) keeps its absolute accuracy
-                            const double dn = dens64(c);
-                            const double vv = kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
-                            const double veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * (dn * sqrt(dn));
-                            c = gpe::phase_rot(c, -a.hdt * veff);
+                            double veff;
+                            if (sizeof(T) == 8 && th0_on) {
+                                veff = (double)th0[q < PRE ? q : 0] + (a.cv * a.cdd) * (double)phi_r[q];
+                            } else {
+                                const double dn = dens64(c);
+                                const double vv =
+                                    kStage ? (double)vpre[q < PRE ? q : 0] : (double)a.V[plane_off + p];
+                                veff = a.cv * (vv + a.cdd * (double)phi_r[q]) + pg * dn + pgm * (dn * sqrt(dn));
+                            }
+                            if constexpr (kStage || sizeof(T) == 4)
+                                c = gpe::phase_rot(c, -a.hdt * veff);
+                            else
+                                c = gpe::phase_rot_lib(c, -a.hdt * veff);
                         }
                         Q[qi] = c;
                     }
