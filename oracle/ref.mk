@@ -12,7 +12,13 @@ SRCS := $(REF)/src/filemanager/file_manager.cpp $(REF)/src/parameters/parameters
 INC := -I$(REF)/src -I$(REF)/src/utils/output -I$(REF)/src/utils/units -I$(REF)/src/utils/types \
        -I$(REF)/lib
 
+ifeq ($(wildcard $(REF)/src),)
+# the reference sources are not on this machine: nothing to build
+all:
+	@true
+else
 all: $(OUT)/gpe_ref_driver
+endif
 
 $(OUT)/gpe_ref_driver: $(SRCS)
 	mkdir -p $(OUT)
