@@ -34,3 +34,36 @@ def test_grid_length_outside_the_radix_set_is_refused(shape):
     d.it_steps(5)
     assert np.isfinite(d.energy()["e_total"])
     d.close()
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (4, 3, 5), (5, 7, 3)])
+@pytest.mark.parametrize("prec", [hip.GPE_FP64, hip.GPE_FP32], ids=["fp64", "fp32"])
+def test_smallest_grids_against_oracle(oracle, shape, prec):
+    """The smallest grids create accepts (every length >= 3; one interior
+    point at 3x3x3) run IT and RT as the oracle does.  50 atoms: a density
+    per point like the other run-time-size tests' (a dense 3x3x3 state
+    amplifies the FP32 rounding as the dense small double wells do)."""
+    nx, ny, nz = shape
+    p = P.PhysicalParameters.from_physical(nx, ny, nz, 120.0, 120.0, 600.0, n_atoms=50.0, edd=1.4)
+    op = oracle.make_params(nx, ny, nz, 120.0, 120.0, 600.0, n_atoms=50.0, edd=1.4)
+    V = P.potential("HARMONIC", p)
+    psi = P.add_phase_noise(P.wavefunction("GAUSS", p), 1e-2, 9)
+    print(f"smallest grid {shape} prec {prec}", flush=True)
+    d = hip.Device(p.to_gpe(), prec)
+    d.load_pote(V)
+    d.load_buffer(psi)
+    o = oracle.Oracle(op)
+    o.load_pote(V)
+    o.load_buffer(psi)
+    tol = 1e-10 if prec == hip.GPE_FP64 else 1e-5
+    d.it_steps(10)
+    o.it_steps(10)
+    e_it = np.linalg.norm(d.export_psi() - o.psi) / np.linalg.norm(o.psi)
+    d.rt_steps(5)
+    o.rt_steps(0, 5)
+    e_rt = np.linalg.norm(d.export_psi() - o.psi) / np.linalg.norm(o.psi)
+    print(f"  IT {e_it:.2e}  RT {e_rt:.2e}", flush=True)
+    assert e_it < tol and e_rt < tol
+    e = d.energy()
+    assert np.isfinite(e["e_total"]) and np.isfinite(e["mu"])
+    d.close()
