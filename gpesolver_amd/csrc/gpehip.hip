@@ -3002,7 +3002,9 @@ struct gpe_handle {
     double t_plane_us = 0, t_line_us = 0, t_total_ms = 0;
 };
 
-static std::string g_create_err;
+// Message of the last failed call that has no handle (create, batch
+// create), per host thread, so concurrent creates do not mix messages.
+static thread_local std::string g_create_err;
 
 // Device address of `scratch` per device, set once per device at gpe_create
 // (with that device current): each device holds its own copy of the array.

@@ -58,7 +58,8 @@ typedef struct {
  * the pencil plane pass, with two extra complex fields of workspace. */
 int gpe_create(const gpe_params *p, int precision, gpe_handle **out);
 void gpe_destroy(gpe_handle *h);
-/* Message of the last failed call on h (h == NULL: last failed gpe_create). */
+/* Message of the last failed call on h (h == NULL: the calling thread's last
+ * failed gpe_create or gpe_batch_create). */
 const char *gpe_last_error(const gpe_handle *h);
 /* Library version string. */
 const char *gpe_version(void);
