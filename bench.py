@@ -41,6 +41,9 @@ Prints ONE JSON line.  Besides steps/s it reports:
                 epochs), then batched IT to |dE/N| < 1e-6 on held-out
                 instances from the FNO's prediction against plain
                 MULTIPLE_GAUSS: wall times (inference included) and steps;
+                `converged_in_both` times both runs again over the instances
+                that reach the tolerance within the 20000-step cap in both
+                (an instance at the cap in both runs ends both runs);
   it_batch_sweep
                 the batched IT sweep (64 instances over 64x64x16, 96x96x24 and
                 128x128x32, each to |dE/N| < 1e-6 or 5000 steps) in one batch
@@ -270,6 +273,7 @@ def warm_vs_plain(n_train: int, n_test: int, epochs: int, seed: int = 20261203) 
         "plain_steps_mean": res["plain_steps_mean"], "warm_steps_mean": res["warm_steps_mean"],
         "plain_steps_max": res["plain_steps_max"], "warm_steps_max": res["warm_steps_max"],
         "step_ratio_median": float(np.median(ps / ws)),
+        "converged_in_both": res["converged_in_both"],
         "fallbacks_before_run": res["fallbacks"], "fallbacks_in_run": res["fallbacks_in_run"],
         "energy_rel_diff_median": res["energy_rel_diff_median"],
         "weights": f"trained in this run: {n_train} generated ground states, {epochs} epochs, batch 8, Adam lr 1e-3 "

@@ -198,6 +198,18 @@ def compare(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID
                                                             fallback=True, warm_model=model)
     # acceptance check: the warm run must reach the plain run's energy
     de = np.abs(e_warm - e_plain) / np.abs(e_plain)
+    # the same comparison over the instances that reach the tolerance within
+    # max_steps in both runs (an instance at the cap in both runs sets both
+    # runs' end and never reaches the stop test), timed again on that set
+    conv = (s_plain < max_steps) & (s_warm < max_steps)
+    both = None
+    if 2 <= int(conv.sum()) < len(insts):
+        sub = [d for d, c in zip(insts, conv) if c]
+        _, _, _, tp_sub = run_batch(sub, [None] * len(sub), tol, max_steps, grid=grid)
+        _, _, _, tw_sub, _, _, _ = run_batch(sub, [None] * len(sub), tol, max_steps, grid=grid, fallback=True,
+                                             warm_model=model)
+        both = {"instances": len(sub), "plain_wall_s": tp_sub, "warm_wall_s": tw_sub,
+                "speedup_wall": tp_sub / tw_sub}
     return {
         "instances": len(insts),
         "plain_wall_s": t_plain,
@@ -212,6 +224,7 @@ def compare(model: F.Fno, insts: list[dict], tol=1e-6, max_steps=5000, grid=GRID
         "fallbacks_in_run": int(taken.sum()),
         "energy_rel_diff_max": float(de.max()),
         "energy_rel_diff_median": float(np.median(de)),
+        "converged_in_both": both,
         "plain_steps": s_plain.tolist(),
         "warm_steps": s_warm.tolist(),
         "plain_e_per_atom": e_plain.tolist(),
