@@ -48,7 +48,9 @@ Prints ONE JSON line.  Besides steps/s it reports:
                 the batched IT sweep (64 instances over 64x64x16, 96x96x24 and
                 128x128x32, each to |dE/N| < 1e-6 or 5000 steps) in one batch
                 with the batch autotuner's x-pass variants, against the same
-                instances one after another on autotuned single handles.
+                instances one after another on autotuned single handles;
+                `batch_step_roofline` is the IT step model summed over every
+                instance's steps, per second of the batched run.
 """
 from __future__ import annotations
 
@@ -238,13 +240,21 @@ def batch_sweep(n: int) -> dict:
     rb = sweep.run_batch(setups, 5000, 100, 1e-6, tune_steps=50)
     rs = sweep.run_sequential(setups, 5000, 100, 1e-6, tune_steps=200)
     de = np.abs(rb["e_per_atom"] - rs["e_per_atom"]) / np.abs(rs["e_per_atom"])
+    # the IT step model (60N + 20H bytes, FP32) summed over every instance's
+    # steps, per second of the batched run
+    model = 0.0
+    for (p, *_), st in zip(setups, rb["steps"]):
+        N, H = p.nx * p.ny * p.nz, 2 * p.nx * 2 * p.ny * (p.nz + 1)
+        model += float(st) * (60 * N + 20 * H)
     return {"instances": n, "grids": [list(s) for s in sweep.SHAPES], "tol_dE_per_N": 1e-6, "max_steps": 5000,
             "batch_s": rb["wall_s"], "batch_tune_s": rb["tune_s"], "sequential_s": rs["wall_s"],
             "speedup": rs["wall_s"] / rb["wall_s"], "batch_converged": rb["converged"],
             "sequential_converged": rs["converged"], "batch_G_point_steps_per_s": rb["G_point_steps_per_s"],
             "sequential_G_point_steps_per_s": rs["G_point_steps_per_s"],
             "steps_mean": float(np.mean(rb["steps"])), "energy_rel_diff_max": float(de.max()),
-            "batch_tuned_per_nx": rb["tuned_per_nx"]}
+            "batch_tuned_per_nx": rb["tuned_per_nx"],
+            "batch_step_roofline": {"model_bytes": model, "achieved_GBps": model / rb["wall_s"] / 1e9,
+                                    "frac": model / rb["wall_s"] / HBM_PEAK}}
 
 
 def warm_vs_plain(n_train: int, n_test: int, epochs: int, seed: int = 20261203) -> dict:
